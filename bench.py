@@ -1,0 +1,273 @@
+"""Benchmark of the GPAW FD grid hot path on one MI355X.
+
+Headline (``value``): batched 6th-order (nn=3, 19-point) FD Laplacian over
+256 real float64 bands on a 96^3 periodic grid, in GPts/s (grid points of
+the whole batch per second), with the kernel's fraction of the 8 TB/s HBM
+roofline.  One step = one Laplace apply over all 256 bands.
+
+Also reported in ``stages`` (same run): Gradient-x nn=3 on the same bands,
+restrict / interpolate nn=3 on 4 densities 96^3 <-> 48^3, the FP64 multigrid
+Poisson solve of the 48^3 four-Gaussian problem (eps 2e-10) and the kinetic
+preconditioner on 32 complex bands.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ('batched FD stencil GPts/s + %HBM peak; '
+          'Poisson time-to-1e-10 mixed vs FP64')
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--no-stages', action='store_true')
+    p.add_argument('--bands', type=int, default=256)
+    return p.parse_args()
+
+
+def timed(fn, steps, warmup):
+    """Average ms per call of fn() between events on torch's current stream
+    (the stream every libgpaw_fd call is launched on)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel), if a summary is committed."""
+    path = os.path.join(ROOT, 'profiles', f'pmc_{name}.md')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith('traffic_bytes_per_launch:'):
+            return float(line.split(':', 1)[1])
+    return None
+
+
+def cpu_baseline(gd, op_coefs, op_offsets, nbands_sample):
+    """The reference C extension (oracle/_ref, built from the reference
+    sources) on the host cores: Laplace nn=3 over a sample of the bands."""
+    threads = len(os.sched_getaffinity(0))
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    so = glob.glob(os.path.join(ROOT, 'oracle', '_ref', '_gpaw_ref*.so'))
+    rng = np.random.default_rng(1)
+    a = rng.uniform(-1, 1, (nbands_sample,) + tuple(gd.n_c))
+    b = np.empty_like(a)
+    n_c = np.ascontiguousarray(gd.n_c)
+    if so:
+        sys.path.insert(0, os.path.dirname(so[0]))
+        import _gpaw_ref
+        r = 3
+        M = n_c + 2 * r
+        off = np.ascontiguousarray(op_offsets @ np.array([M[1] * M[2], M[2],
+                                                          1]))
+        op = _gpaw_ref.Operator(np.ascontiguousarray(op_coefs), off, n_c, r,
+                                np.ascontiguousarray(gd.neighbor_cd), 1, None,
+                                1)
+        run = lambda: op.apply(a, b)  # noqa: E731
+        kind = 'reference'
+    else:
+        import oracle
+        run = lambda: oracle.fd_apply(op_coefs, op_offsets, n_c,  # noqa
+                                      gd.neighbor_cd, a, out=b,
+                                      nthreads=threads)
+        kind = 'port'
+    run()
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        run()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > 8.0 or reps >= 50:
+            break
+    pts = reps * a.size
+    return {'value': round(pts / el / 1e9, 4), 'unit': 'GPts/s',
+            'cores': threads, 'kind': kind,
+            'sample': f'{nbands_sample} of the 256 bands (96^3, f64), '
+                      f'{reps} applies, {el:.1f} s, OpenMP over bands'}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; '
+                          f'--gpus {args.gpus} is not supported'}))
+        sys.exit(2)
+    import torch
+    import gpaw_amd
+    from gpaw_amd import _lib, workloads
+    from gpaw_amd.fd_operators import Gradient, Laplace
+    _lib.load()
+    torch.cuda.init()
+    dev = torch.device('cuda', 0)
+
+    gd = workloads.bands_96_gd()
+    nb = args.bands
+    lap = Laplace(gd, 1.0, 3)
+    log(f'[bench] {nb} bands 96^3 f64, Laplace nn=3 ({lap.npoints} points)')
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    a = torch.rand((nb, 96, 96, 96), dtype=torch.float64, device=dev,
+                   generator=g) * 2 - 1
+    b = torch.empty_like(a)
+    ms, wall = timed(lambda: lap.apply(a, b), args.steps, args.warmup)
+    pts = nb * 96**3
+    value = pts / (ms * 1e-3) / 1e9
+    alg_bytes = pts * 16
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    log(f'[bench] laplace: {ms:.4f} ms/step (wall {wall:.4f}), '
+        f'{value:.1f} GPts/s, {achieved:.0f} GB/s')
+
+    # parity spot checks at full size against the oracle + a size-free
+    # property (the periodic Laplacian of any field sums to zero)
+    import oracle
+    parity = {}
+    for band in (0, nb - 1):
+        ref = oracle.fd_apply(lap.coef_p, lap.offset_pc, gd.n_c,
+                              gd.neighbor_cd, a[band].cpu().numpy())
+        got = b[band].cpu().numpy()
+        parity[f'band{band}_relerr'] = float(
+            np.abs(got - ref).max() / np.abs(ref).max())
+    s = b.sum(dim=(1, 2, 3)).abs().max().item()
+    parity['max_band_sum_over_l1'] = s / b.abs().sum(dim=(1, 2, 3)).min().item()
+    parity['ok'] = bool(max(parity['band0_relerr'],
+                            parity[f'band{nb - 1}_relerr']) < 1e-13
+                        and parity['max_band_sum_over_l1'] < 1e-12)
+    log('[bench] parity', parity)
+
+    stages = {}
+    if not args.no_stages:
+        gx = Gradient(gd, 0, 1.0, 3)
+        ms_g, _ = timed(lambda: gx.apply(a, b), max(5, args.steps // 2), 2)
+        stages['gradient_x_nn3_bands_96'] = {
+            'ms': round(ms_g, 4), 'GPts/s': round(pts / ms_g / 1e6, 1),
+            'hbm_frac': round(alg_bytes / ms_g / 1e6 / HBM_PEAK_GBS, 4)}
+        del a, b
+        torch.cuda.empty_cache()
+        from gpaw_amd.transformers import Transformer
+        gdd, dens = workloads.density_96()
+        coarse = gdd.coarsen()
+        R = Transformer(gdd, coarse, 3)
+        I = Transformer(coarse, gdd, 3)
+        d = torch.from_numpy(dens).to(dev)
+        c = torch.empty((4, 48, 48, 48), dtype=torch.float64, device=dev)
+        f = torch.empty_like(d)
+        ms_r, _ = timed(lambda: R.apply(d, c), 20, 3)
+        ms_i, _ = timed(lambda: I.apply(c, f), 20, 3)
+        fine_pts = 4 * 96**3
+        stages['restrict_nn3_density_96'] = {
+            'ms': round(ms_r, 4),
+            'GPts/s_fine': round(fine_pts / ms_r / 1e6, 1),
+            'note': 'working set 36 MB: Infinity-Cache resident'}
+        stages['interpolate_nn3_density_96'] = {
+            'ms': round(ms_i, 4),
+            'GPts/s_fine': round(fine_pts / ms_i / 1e6, 1)}
+        from gpaw_amd.poisson import FDPoissonSolver
+        gdp, rho = workloads.poisson_48()
+        ps = FDPoissonSolver(nn=3, eps=2e-10)
+        ps.set_grid_descriptor(gdp)
+        drho = torch.from_numpy(rho).to(dev)
+        phi = torch.zeros_like(drho)
+        ps.solve(phi, drho)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(3):
+            phi.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            niter = ps.solve(phi, drho)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        stages['poisson_fp64_48'] = {
+            'ms_to_eps': round(min(times) * 1e3, 3), 'niter': niter,
+            'eps': 2e-10, 'error': ps.last_error}
+        from gpaw_amd.preconditioner import Preconditioner
+        L = 128 * 0.18 / gpaw_amd.Bohr
+        from gpaw_amd.grid_descriptor import GridDescriptor
+        gdk = GridDescriptor((128, 128, 128), (L, L, L))
+        kin = Laplace(gdk, -0.5, 3, complex)
+        P = Preconditioner(gdk, kin, complex, block=32)
+        res = torch.rand((32, 128, 128, 128), dtype=torch.complex128,
+                         device=dev)
+        out = torch.empty_like(res)
+
+        class Kpt:
+            phase_cd = np.exp(2j * np.pi * np.array([[0.1], [0.0], [0.25]]) *
+                              np.array([1, -1]))
+        ms_p, _ = timed(lambda: P(res, Kpt, out=out), 5, 1)
+        stages['precond_c128_32x128'] = {
+            'ms': round(ms_p, 3),
+            'GPts/s': round(32 * 128**3 / ms_p / 1e6, 1)}
+        log('[bench] stages', stages)
+
+    cpu = None
+    if not args.no_cpu:
+        log('[bench] CPU baseline ...')
+        cpu = cpu_baseline(gd, lap.coef_p, lap.offset_pc, 16)
+        log('[bench] cpu', cpu)
+
+    traffic = traffic_from_profiles('laplace_nn3_bands_96')
+    line = {
+        'metric': METRIC,
+        'value': round(value, 2),
+        'unit': 'GPts/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (uniform [-1, 1] bands, seeded)',
+        'config': {'workload': 'laplace_nn3_bands_96', 'bands': nb,
+                   'grid': [96, 96, 96], 'h_angstrom': 0.2,
+                   'stencil': '19-point nn=3 (O(h^6))',
+                   'boundary': 'periodic', 'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1),
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4),
+                     'traffic': traffic},
+        'cpu_baseline': cpu,
+        'parity': parity,
+        'stages': stages,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,114 @@
+"""ctypes binding of libgpaw_fd.so (the C ABI declared in include/gpaw_fd.h).
+
+The library is the only compute path of this package: if it is missing, or
+no GPU is visible when a kernel is called, the call raises -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libgpaw_fd.so')
+
+GFD_OK = 0
+GFD_EINVAL = 1
+GFD_EHIP = 2
+GFD_ENOMEM = 3
+GFD_ENOTSUP = 4
+GFD_ECONVERGE = 5
+
+# every symbol of include/gpaw_fd.h with (restype, argtypes)
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i = ctypes.c_int
+_l = ctypes.c_long
+_d = ctypes.c_double
+
+SIGNATURES = {
+    'gfd_last_error': (ctypes.c_char_p, []),
+    'gfd_abi_version': (_i, []),
+    'gfd_operator_new': (_i, [_i, _dp, _lp, _lp, _i, _lp, _i, _i,
+                              ctypes.POINTER(_vp)]),
+    'gfd_operator_free': (None, [_vp]),
+    'gfd_operator_apply': (_i, [_vp, _vp, _vp, _l, _dp, _vp]),
+    'gfd_operator_relax': (_i, [_vp, _i, _vp, _vp, _i, _d, _vp]),
+    'gfd_operator_get_diagonal_element': (_d, [_vp]),
+    'gfd_operator_residual': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _dp, _vp]),
+    'gfd_transformer_new': (_i, [_lp, _lp, _i, _lp, _lp, _lp, _lp, _i, _i,
+                                 ctypes.POINTER(_vp)]),
+    'gfd_transformer_free': (None, [_vp]),
+    'gfd_transformer_apply': (_i, [_vp, _vp, _vp, _l, _dp, _vp]),
+    'gfd_fd_precond': (_i, [_vp] * 7 + [_vp] * 8 + [_vp, _l, _d, _dp, _vp]),
+    'gfd_poisson_new': (_i, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                             ctypes.POINTER(_vp), _ip, _ip, _dp, _i, _d,
+                             ctypes.POINTER(_vp)]),
+    'gfd_poisson_free': (None, [_vp]),
+    'gfd_poisson_iterate': (_i, [_vp, _vp, _vp, _dp, _vp]),
+    'gfd_poisson_solve': (_i, [_vp, _vp, _vp, _d, _i, _i, _ip, _dp, _vp]),
+    'gfd_dot': (_i, [_vp, _vp, _l, _dp, _vp]),
+    'gfd_add_scalar': (_i, [_vp, _l, _d, _vp]),
+    'gfd_set_tuning': (_i, [ctypes.c_char_p, _l]),
+}
+
+_lib = None
+
+
+class GPUError(RuntimeError):
+    """A call into libgpaw_fd.so failed."""
+
+    def __init__(self, status, message):
+        super().__init__(f'[gfd status {status}] {message}')
+        self.status = status
+
+
+def load():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f'{LIB_PATH} is missing: build it with '
+                '`python -m gpaw_amd._build` (hipcc, gfx950)')
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status, what=''):
+    if status != GFD_OK:
+        msg = load().gfd_last_error().decode(errors='replace')
+        if status == GFD_EINVAL:
+            raise ValueError(f'{what}: {msg}')
+        if status == GFD_ENOTSUP:
+            raise NotImplementedError(f'{what}: {msg}')
+        raise GPUError(status, f'{what}: {msg}')
+
+
+def longs(x):
+    """Contiguous C long array (keeps a reference via the returned tuple)."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int64).ravel())
+    return a, a.ctypes.data_as(_lp)
+
+
+def doubles(x):
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel())
+    return a, a.ctypes.data_as(_dp)
+
+
+def phases_arg(phase_cd):
+    """phase_cd (3, 2) complex -> 12 doubles, or NULL."""
+    import numpy as np
+    if phase_cd is None:
+        return None, None
+    ph = np.ascontiguousarray(np.asarray(phase_cd, dtype=complex).reshape(3, 2))
+    return ph, ph.view(np.float64).ctypes.data_as(_dp)

@@ -1,0 +1,931 @@
+// C ABI of libgpaw_fd.so: handles, argument checking and the native
+// multigrid drivers (Poisson V-cycle, kinetic preconditioner chain).
+#include "../../include/gpaw_fd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gfd_common.h"
+#include "stencil.h"
+#include "transfer.h"
+#include "vector.h"
+
+namespace gfd {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+static bool g_force_general = false;
+
+// Grow-only device buffer.  Growing waits for the stream first, so a buffer
+// still read by queued kernels is never freed under them.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want, hipStream_t s)
+    {
+        if (want <= bytes) return OK;
+        if (p) {
+            GFD_CHECK_HIP(hipStreamSynchronize(s));
+            GFD_CHECK_HIP(hipFree(p));
+            p = nullptr;
+            bytes = 0;
+        }
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            set_error("hipMalloc of %zu bytes failed", want);
+            return ENOMEM_;
+        }
+        bytes = want;
+        return OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+static BC make_bc(const int periodic[3], const double* phases)
+{
+    BC bc;
+    for (int c = 0; c < 3; c++) {
+        bc.periodic[c] = periodic[c];
+        for (int d = 0; d < 2; d++)
+            bc.phase[c][d] = phases ? make_double2(phases[4 * c + 2 * d],
+                                                   phases[4 * c + 2 * d + 1])
+                                    : make_double2(1.0, 0.0);
+    }
+    return bc;
+}
+
+static int periodic_from_neighbors(const long nb[6], int periodic[3])
+{
+    for (int c = 0; c < 3; c++) {
+        const bool lo = nb[2 * c] >= 0, hi = nb[2 * c + 1] >= 0;
+        if (lo != hi) {
+            set_error("axis %d: mixed periodic / zero sides are not supported "
+                      "on a single domain",
+                      c);
+            return ENOTSUP_;
+        }
+        periodic[c] = lo ? 1 : 0;
+    }
+    return OK;
+}
+
+}  // namespace gfd
+
+using namespace gfd;
+
+// ====================================================================== ops
+
+struct gfd_operator {
+    int P = 0;
+    std::vector<double> coef;
+    std::vector<int> off;  // P x 3
+    int n[3] = {0, 0, 0};
+    int range = 0;
+    int periodic[3] = {0, 0, 0};
+    int real = 1;
+    double diag = 0.0;      // get_diagonal_element (last zero offset)
+    int centre = -1;        // first zero offset
+    bool axis_ok = false;
+    AxisStencil ax{};
+    double* d_coef = nullptr;
+    int* d_off = nullptr;
+    DevBuf scratch;         // relax ping-pong / in-place copies
+    DevBuf partial;         // residual-norm partial sums
+    DevBuf scalar;          // one device double
+    long ng() const { return (long)n[0] * n[1] * n[2]; }
+    int dtype() const { return real ? 0 : 1; }
+    GeneralStencil gen(double c0) const
+    {
+        GeneralStencil g;
+        g.P = P;
+        g.centre = centre;
+        g.c0 = c0;
+        g.coef = d_coef;
+        g.off = d_off;
+        return g;
+    }
+    bool use_fast() const { return axis_ok && !g_force_general; }
+};
+
+extern "C" const char* gfd_last_error(void) { return gfd::last_error(); }
+
+extern "C" int gfd_abi_version(void) { return 1; }
+
+extern "C" int gfd_operator_new(int ncoefs, const double* coefs,
+                                const long* offsets, const long size[3],
+                                int range, const long neighbors[6], int real,
+                                int cfd, gfd_operator** out)
+{
+    (void)cfd;
+    if (!out || ncoefs <= 0 || ncoefs > MAXP || !coefs || !offsets || !size ||
+        !neighbors || range < 0) {
+        set_error("gfd_operator_new: bad arguments");
+        return EINVAL_;
+    }
+    *out = nullptr;
+    gfd_operator* op = new gfd_operator();
+    op->P = ncoefs;
+    op->range = range;
+    op->real = real ? 1 : 0;
+    for (int c = 0; c < 3; c++) {
+        if (size[c] <= 0 || size[c] > (1L << 20)) {
+            delete op;
+            set_error("gfd_operator_new: bad grid size");
+            return EINVAL_;
+        }
+        op->n[c] = (int)size[c];
+    }
+    int st = periodic_from_neighbors(neighbors, op->periodic);
+    if (st != OK) {
+        delete op;
+        return st;
+    }
+    // decode the padded-stride offsets (gpaw/fd_operators.py:61-63)
+    const long M1 = size[1] + 2 * range, M2 = size[2] + 2 * range;
+    const long shift = range * (M1 * M2 + M2 + 1);
+    op->coef.assign(coefs, coefs + ncoefs);
+    op->off.resize(3 * ncoefs);
+    for (int p = 0; p < ncoefs; p++) {
+        const long t = offsets[p] + shift;
+        if (t < 0) {
+            delete op;
+            set_error("gfd_operator_new: offset %ld outside range %d",
+                      offsets[p], range);
+            return EINVAL_;
+        }
+        const long o2 = t % M2 - range;
+        const long o1 = (t / M2) % M1 - range;
+        const long o0 = t / (M1 * M2) - range;
+        if (o0 < -range || o0 > range) {
+            delete op;
+            set_error("gfd_operator_new: offset %ld outside range %d",
+                      offsets[p], range);
+            return EINVAL_;
+        }
+        const long oc[3] = {o0, o1, o2};
+        for (int c = 0; c < 3; c++)
+            if (oc[c] > size[c] || -oc[c] > size[c]) {
+                delete op;
+                set_error("Stencil longer than domain");
+                return EINVAL_;
+            }
+        op->off[3 * p] = (int)o0;
+        op->off[3 * p + 1] = (int)o1;
+        op->off[3 * p + 2] = (int)o2;
+        if (o0 == 0 && o1 == 0 && o2 == 0) {
+            op->diag = coefs[p];
+            if (op->centre < 0) op->centre = p;
+        }
+    }
+    // per-axis form for the x-march kernel
+    AxisStencil& ax = op->ax;
+    std::memset(&ax, 0, sizeof ax);
+    bool ok = true;
+    int R = 0, ncentre = 0;
+    std::vector<int> seen(3 * (2 * MAXR_FAST + 1), 0);
+    for (int p = 0; p < ncoefs && ok; p++) {
+        const int* o = &op->off[3 * p];
+        const int nz = (o[0] != 0) + (o[1] != 0) + (o[2] != 0);
+        if (nz == 0) {
+            ax.c0 = coefs[p];
+            ncentre++;
+            continue;
+        }
+        if (nz > 1) {
+            ok = false;
+            break;
+        }
+        const int a = o[0] ? 0 : (o[1] ? 1 : 2);
+        const int k = o[a];
+        if (k > MAXR_FAST || -k > MAXR_FAST) {
+            ok = false;
+            break;
+        }
+        int& sflag = seen[a * (2 * MAXR_FAST + 1) + k + MAXR_FAST];
+        if (sflag) {
+            ok = false;  // duplicate offset: keep separate terms (general)
+            break;
+        }
+        sflag = 1;
+        if (k > 0) ax.cp[a][k - 1] = coefs[p];
+        else ax.cm[a][-k - 1] = coefs[p];
+        ax.has[a] = 1;
+        R = std::max(R, std::abs(k));
+    }
+    if (ncentre > 1 || (ncentre == 1 && op->centre != 0 && false)) ok = false;
+    if (R == 0) ok = false;
+    for (int c = 0; c < 3; c++)
+        if (op->n[c] < R) ok = false;
+    ax.R = R;
+    op->axis_ok = ok;
+    // device copies for the general kernel
+    if (hipMalloc((void**)&op->d_coef, sizeof(double) * ncoefs) != hipSuccess ||
+        hipMalloc((void**)&op->d_off, sizeof(int) * 3 * ncoefs) != hipSuccess) {
+        (void)hipGetLastError();
+        if (op->d_coef) (void)hipFree(op->d_coef);
+        delete op;
+        set_error("gfd_operator_new: device allocation failed (no GPU?)");
+        return ENOMEM_;
+    }
+    if (hipMemcpy(op->d_coef, op->coef.data(), sizeof(double) * ncoefs,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(op->d_off, op->off.data(), sizeof(int) * 3 * ncoefs,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(op->d_coef);
+        (void)hipFree(op->d_off);
+        delete op;
+        set_error("gfd_operator_new: copy of coefficients failed");
+        return EHIP;
+    }
+    *out = op;
+    return OK;
+}
+
+extern "C" void gfd_operator_free(gfd_operator* op)
+{
+    if (!op) return;
+    if (op->d_coef) (void)hipFree(op->d_coef);
+    if (op->d_off) (void)hipFree(op->d_off);
+    op->scratch.release();
+    op->partial.release();
+    op->scalar.release();
+    delete op;
+}
+
+extern "C" double gfd_operator_get_diagonal_element(const gfd_operator* op)
+{
+    return op ? op->diag : 0.0;
+}
+
+// One stencil pass with an epilogue (the building block of every driver).
+static int op_pass(gfd_operator* op, int dtype, const void* in, void* out,
+                   long nin, const double* phases, int epi, EpiArgs ea,
+                   hipStream_t s, int* nblocks)
+{
+    const BC bc = make_bc(op->periodic, phases);
+    if (op->use_fast())
+        return launch_axis_stencil(op->ax, dtype, op->n, bc, in, out, nin, epi,
+                                   ea, s, nblocks);
+    return launch_general_stencil(op->gen(ea.c0), dtype, op->n, bc, in, out,
+                                  nin, epi, ea, s, nblocks);
+}
+
+static long op_max_blocks(gfd_operator* op, int dtype, long nin)
+{
+    if (op->use_fast())
+        return axis_stencil_max_blocks(op->ax, dtype, op->n, nin);
+    return general_stencil_max_blocks(op->n, nin);
+}
+
+static double centre_coef(const gfd_operator* op)
+{
+    return op->centre >= 0 ? op->coef[op->centre] : 0.0;
+}
+
+extern "C" int gfd_operator_apply(gfd_operator* op, const void* in, void* out,
+                                  long nin, const double* phases, void* stream)
+{
+    if (!op || nin < 0 || (nin > 0 && (!in || !out))) {
+        set_error("gfd_operator_apply: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) return OK;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t es = op->real ? 8 : 16;
+    const void* src = in;
+    if (in == out) {  // the reference copies into a padded buffer first
+        GFD_TRY(op->scratch.ensure(es * op->ng() * nin, s));
+        GFD_CHECK_HIP(hipMemcpyAsync(op->scratch.p, in, es * op->ng() * nin,
+                                     hipMemcpyDeviceToDevice, s));
+        src = op->scratch.p;
+    }
+    EpiArgs ea{};
+    ea.c0 = centre_coef(op);
+    return op_pass(op, op->dtype(), src, out, nin, op->real ? nullptr : phases,
+                   EPI_PLAIN, ea, s, nullptr);
+}
+
+// nrelax Jacobi sweeps ping-ponging between *cur and *alt; on return *cur
+// holds the result (the pointers are swapped once per sweep).
+static int relax_pp(gfd_operator* op, double** cur, double** alt,
+                    const double* src, int nrelax, double w, hipStream_t s)
+{
+    EpiArgs ea{};
+    ea.aux = src;
+    ea.c0 = op->coef[0];
+    ea.w = w;
+    for (int k = 0; k < nrelax; k++) {
+        GFD_TRY(op_pass(op, 0, *cur, *alt, 1, nullptr, EPI_RELAX, ea, s,
+                        nullptr));
+        std::swap(*cur, *alt);
+    }
+    return OK;
+}
+
+static int check_relax(const gfd_operator* op, int method)
+{
+    if (!op->real) {
+        set_error("relax: real data only");
+        return EINVAL_;
+    }
+    if (method == 1) {
+        set_error("relax: Gauss-Seidel (method 1) is not implemented on the "
+                  "GPU; use Jacobi (method 2)");
+        return ENOTSUP_;
+    }
+    if (method != 2) {
+        set_error("relax: unknown method %d", method);
+        return EINVAL_;
+    }
+    if (op->centre != 0) {
+        set_error("relax: the first stencil point must be the centre");
+        return EINVAL_;
+    }
+    return OK;
+}
+
+extern "C" int gfd_operator_relax(gfd_operator* op, int method, double* f,
+                                  const double* src, int nrelax, double w,
+                                  void* stream)
+{
+    if (!op || !f || !src || nrelax < 0) {
+        set_error("gfd_operator_relax: bad arguments");
+        return EINVAL_;
+    }
+    GFD_TRY(check_relax(op, method));
+    if (nrelax == 0) return OK;
+    hipStream_t s = (hipStream_t)stream;
+    const long ng = op->ng();
+    GFD_TRY(op->scratch.ensure(2 * sizeof(double) * ng, s));
+    double* t1 = (double*)op->scratch.p;
+    double* t2 = t1 + ng;
+    // f -> t1 -> t2 -> ... -> f: the last sweep writes f directly
+    double* cur = f;
+    EpiArgs ea{};
+    ea.aux = src;
+    ea.c0 = op->coef[0];
+    ea.w = w;
+    for (int k = 0; k < nrelax; k++) {
+        double* dst;
+        if (k == nrelax - 1 && cur != f) dst = f;
+        else dst = (cur == t1) ? t2 : t1;
+        GFD_TRY(op_pass(op, 0, cur, dst, 1, nullptr, EPI_RELAX, ea, s, nullptr));
+        cur = dst;
+    }
+    if (cur != f)
+        GFD_CHECK_HIP(hipMemcpyAsync(f, cur, sizeof(double) * ng,
+                                     hipMemcpyDeviceToDevice, s));
+    return OK;
+}
+
+static int residual_norm(gfd_operator* op, int dtype, const void* in,
+                         const void* b, void* r, long nin,
+                         const double* phases, double* dev_norm,
+                         hipStream_t s)
+{
+    EpiArgs ea{};
+    ea.aux = b;
+    ea.c0 = centre_coef(op);
+    if (dev_norm) {
+        const long nbmax = op_max_blocks(op, dtype, nin);
+        GFD_TRY(op->partial.ensure(sizeof(double) * (nbmax + 1), s));
+        ea.partial = (double*)op->partial.p;
+    }
+    int nb = 0;
+    GFD_TRY(op_pass(op, dtype, in, r, nin, phases, EPI_RESID, ea, s, &nb));
+    if (dev_norm) GFD_TRY(launch_sum_partials(ea.partial, nb, dev_norm, s));
+    return OK;
+}
+
+extern "C" int gfd_operator_residual(gfd_operator* op, const void* in,
+                                     const void* b, void* r, long nin,
+                                     const double* phases, double* norm2,
+                                     void* stream)
+{
+    if (!op || nin < 0 || (nin > 0 && (!in || !b || !r)) || in == r) {
+        set_error("gfd_operator_residual: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) {
+        if (norm2) *norm2 = 0.0;
+        return OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    double* dn = nullptr;
+    if (norm2) {
+        GFD_TRY(op->scalar.ensure(sizeof(double), s));
+        dn = (double*)op->scalar.p;
+    }
+    GFD_TRY(residual_norm(op, op->dtype(), in, b, r, nin,
+                          op->real ? nullptr : phases, dn, s));
+    if (norm2) {
+        GFD_CHECK_HIP(hipMemcpyAsync(norm2, dn, sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        GFD_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return OK;
+}
+
+// ================================================================ transfers
+
+struct gfd_transformer {
+    XferPlan plan{};
+    int periodic[3] = {0, 0, 0};
+    int real = 1;
+    DevBuf scratch;
+    long ngin() const
+    {
+        return (long)plan.nin[0] * plan.nin[1] * plan.nin[2];
+    }
+    long ngout() const
+    {
+        return (long)plan.nout[0] * plan.nout[1] * plan.nout[2];
+    }
+    int dtype() const { return real ? 0 : 1; }
+};
+
+extern "C" int gfd_transformer_new(const long size_in[3],
+                                   const long size_out[3], int k,
+                                   const long pad[6], const long npad[6],
+                                   const long skip[6],
+                                   const long neighbors[6], int real,
+                                   int interpolate, gfd_transformer** out)
+{
+    if (!out || !size_in || !size_out || !pad || !npad || !skip ||
+        !neighbors || (k != 2 && k != 4 && k != 6 && k != 8)) {
+        set_error("gfd_transformer_new: bad arguments (k must be 2,4,6,8)");
+        return EINVAL_;
+    }
+    *out = nullptr;
+    gfd_transformer* t = new gfd_transformer();
+    t->real = real ? 1 : 0;
+    int st = periodic_from_neighbors(neighbors, t->periodic);
+    if (st != OK) {
+        delete t;
+        return st;
+    }
+    XferPlan& p = t->plan;
+    p.interpolate = interpolate ? 1 : 0;
+    p.K = k;
+    for (int c = 0; c < 3; c++) {
+        p.nin[c] = (int)size_in[c];
+        p.nout[c] = (int)size_out[c];
+        const long p0 = pad[2 * c], p1 = pad[2 * c + 1];
+        if (p0 < 0 || p1 < 0 || size_in[c] <= 0 || size_out[c] <= 0) {
+            delete t;
+            set_error("gfd_transformer_new: negative padding or empty grid");
+            return EINVAL_;
+        }
+        // a periodic halo is a pure wrap only when each side receives what
+        // the other side sends (c/bc.cpp:66-78)
+        if (t->periodic[c] && (npad[2 * c] != p1 || npad[2 * c + 1] != p0)) {
+            delete t;
+            set_error("gfd_transformer_new: axis %d paddings are not a "
+                      "single-domain wrap",
+                      c);
+            return ENOTSUP_;
+        }
+        const long M = size_in[c] + p0 + p1;
+        XferAxis& a = p.ax[c];
+        a.nin = (int)size_in[c];
+        a.nout = (int)size_out[c];
+        if (!interpolate) {
+            const long nout = (M - (2 * k - 3)) / 2;
+            if (nout != size_out[c]) {
+                delete t;
+                set_error("gfd_transformer_new: axis %d restricts %ld -> %ld, "
+                          "not %ld",
+                          c, size_in[c], nout, size_out[c]);
+                return EINVAL_;
+            }
+            a.off = (int)((k - 1) - p0);
+            a.skip0 = 0;
+        } else {
+            const long s0 = skip[2 * c], s1 = skip[2 * c + 1];
+            const long nloop = M - (k - 1) + s1;
+            const long nout = 2 * nloop - s0 - s1;
+            if (nout != size_out[c]) {
+                delete t;
+                set_error("gfd_transformer_new: axis %d interpolates %ld -> "
+                          "%ld, not %ld",
+                          c, size_in[c], nout, size_out[c]);
+                return EINVAL_;
+            }
+            a.off = (int)((k / 2 - 1) - p0);
+            a.skip0 = (int)s0;
+        }
+    }
+    *out = t;
+    return OK;
+}
+
+extern "C" void gfd_transformer_free(gfd_transformer* t)
+{
+    if (!t) return;
+    t->scratch.release();
+    delete t;
+}
+
+static int xfer(gfd_transformer* t, const void* in, void* out, long nin,
+                const double* phases, const XEpiArgs& ea, hipStream_t s)
+{
+    const size_t es = t->real ? 8 : 16;
+    GFD_TRY(t->scratch.ensure(es * xfer_scratch_elems(t->plan, nin) + 16, s));
+    const BC bc = make_bc(t->periodic, t->real ? nullptr : phases);
+    return launch_transfer(t->plan, t->dtype(), bc, in, out, nin, t->scratch.p,
+                           ea, s);
+}
+
+extern "C" int gfd_transformer_apply(gfd_transformer* t, const void* in,
+                                     void* out, long nin,
+                                     const double* phases, void* stream)
+{
+    if (!t || nin < 0 || (nin > 0 && (!in || !out))) {
+        set_error("gfd_transformer_apply: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) return OK;
+    XEpiArgs ea;
+    return xfer(t, in, out, nin, phases, ea, (hipStream_t)stream);
+}
+
+// ============================================================ preconditioner
+
+static bool same3(const int a[3], const int b[3])
+{
+    return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
+                              gfd_transformer* intp1, gfd_transformer* intp2,
+                              gfd_operator* kin0, gfd_operator* kin1,
+                              gfd_operator* kin2, void* d0, void* q0, void* r1,
+                              void* d1, void* q1, void* r2, void* d2, void* q2,
+                              const void* residuals, long nin, double step,
+                              const double* phases, void* stream)
+{
+    if (!rest0 || !rest1 || !intp1 || !intp2 || !kin0 || !kin1 || !kin2 ||
+        nin < 0) {
+        set_error("gfd_fd_precond: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) return OK;
+    if (!d0 || !q0 || !r1 || !d1 || !q1 || !r2 || !d2 || !q2 || !residuals) {
+        set_error("gfd_fd_precond: null work array");
+        return EINVAL_;
+    }
+    const int real = kin0->real;
+    if (kin1->real != real || kin2->real != real || rest0->real != real ||
+        rest1->real != real || intp1->real != real || intp2->real != real) {
+        set_error("gfd_fd_precond: mixed real/complex objects");
+        return EINVAL_;
+    }
+    if (rest0->plan.interpolate || rest1->plan.interpolate ||
+        !intp1->plan.interpolate || !intp2->plan.interpolate ||
+        !same3(rest0->plan.nin, kin0->n) || !same3(rest0->plan.nout, kin1->n) ||
+        !same3(rest1->plan.nin, kin1->n) || !same3(rest1->plan.nout, kin2->n) ||
+        !same3(intp2->plan.nin, kin2->n) || !same3(intp2->plan.nout, kin1->n) ||
+        !same3(intp1->plan.nin, kin1->n) || !same3(intp1->plan.nout, kin0->n)) {
+        set_error("Input grid shape does not match with arrays d0 and q0.");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const double* ph = real ? nullptr : phases;
+    const int dt = real ? 0 : 1;
+    // r1 = R0(-res); d1 = 4 step r1
+    {
+        XEpiArgs e;
+        e.epi = XEPI_SCALE2;
+        e.negate = 1;
+        e.alpha = 4 * step;
+        e.out2 = d1;
+        GFD_TRY(xfer(rest0, residuals, r1, nin, ph, e, s));
+    }
+    // q1 = K1 d1 - r1
+    {
+        EpiArgs e{};
+        e.aux = r1;
+        e.c0 = centre_coef(kin1);
+        GFD_TRY(op_pass(kin1, dt, d1, q1, nin, ph, EPI_RESID, e, s, nullptr));
+    }
+    // r2 = R1(q1); d2 = 16 step r2
+    {
+        XEpiArgs e;
+        e.epi = XEPI_SCALE2;
+        e.alpha = 16 * step;
+        e.out2 = d2;
+        GFD_TRY(xfer(rest1, q1, r2, nin, ph, e, s));
+    }
+    // q2 = K2 d2 - r2; d2 -= 16 step q2        (new d2 held in q2)
+    {
+        EpiArgs e{};
+        e.aux = r2;
+        e.c0 = centre_coef(kin2);
+        e.s = 16 * step;
+        GFD_TRY(op_pass(kin2, dt, d2, q2, nin, ph, EPI_UPDATE, e, s, nullptr));
+    }
+    // q1 = I2(d2); d1 -= q1
+    {
+        XEpiArgs e;
+        e.epi = XEPI_SUBFROM;
+        e.base = d1;
+        GFD_TRY(xfer(intp2, q2, d1, nin, ph, e, s));
+    }
+    // q1 = K1 d1 - r1; d1 -= 4 step q1          (new d1 held in q1)
+    {
+        EpiArgs e{};
+        e.aux = r1;
+        e.c0 = centre_coef(kin1);
+        e.s = 4 * step;
+        GFD_TRY(op_pass(kin1, dt, d1, q1, nin, ph, EPI_UPDATE, e, s, nullptr));
+    }
+    // d0' = I1(-d1)                              (held in q0)
+    {
+        XEpiArgs e;
+        e.epi = XEPI_NEG;
+        GFD_TRY(xfer(intp1, q1, q0, nin, ph, e, s));
+    }
+    // q0 = K0 d0' - res; d0 = -(d0' - step q0)
+    {
+        EpiArgs e{};
+        e.aux = residuals;
+        e.c0 = centre_coef(kin0);
+        e.s = step;
+        e.negate = 1;
+        GFD_TRY(op_pass(kin0, dt, q0, d0, nin, ph, EPI_UPDATE, e, s, nullptr));
+    }
+    return OK;
+}
+
+// ================================================================= Poisson
+
+struct gfd_poisson {
+    int L = 0;  // number of coarsenings
+    std::vector<gfd_operator*> ops;
+    std::vector<gfd_transformer*> rst, itp;
+    std::vector<int> pre, post;
+    std::vector<double> w;
+    double dv = 1.0;
+    DevBuf pool;  // phi, tmp, rho, residual of every level
+    std::vector<double*> phi, tmp, rho, res;
+    DevBuf scalar;
+};
+
+extern "C" int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
+                               gfd_transformer* const* restrictors,
+                               gfd_transformer* const* interpolators,
+                               const int* presmooths, const int* postsmooths,
+                               const double* weights, int relax_method,
+                               double dv, gfd_poisson** out)
+{
+    if (!out || nlevels < 0 || !ops || (nlevels > 0 && (!restrictors ||
+                                                        !interpolators)) ||
+        !presmooths || !postsmooths || !weights) {
+        set_error("gfd_poisson_new: bad arguments");
+        return EINVAL_;
+    }
+    *out = nullptr;
+    for (int l = 0; l <= nlevels; l++) {
+        if (!ops[l]) {
+            set_error("gfd_poisson_new: null operator");
+            return EINVAL_;
+        }
+        int st = check_relax(ops[l], relax_method);
+        if (st != OK) return st;
+    }
+    for (int l = 0; l < nlevels; l++) {
+        gfd_transformer* r = restrictors[l];
+        gfd_transformer* i = interpolators[l];
+        if (!r || !i || r->plan.interpolate || !i->plan.interpolate ||
+            !r->real || !i->real || !same3(r->plan.nin, ops[l]->n) ||
+            !same3(r->plan.nout, ops[l + 1]->n) ||
+            !same3(i->plan.nin, ops[l + 1]->n) ||
+            !same3(i->plan.nout, ops[l]->n)) {
+            set_error("gfd_poisson_new: level %d transformers do not match", l);
+            return EINVAL_;
+        }
+    }
+    gfd_poisson* p = new gfd_poisson();
+    p->L = nlevels;
+    p->ops.assign(ops, ops + nlevels + 1);
+    if (nlevels > 0) {
+        p->rst.assign(restrictors, restrictors + nlevels);
+        p->itp.assign(interpolators, interpolators + nlevels);
+    }
+    p->pre.assign(presmooths, presmooths + nlevels + 1);
+    p->post.assign(postsmooths, postsmooths + nlevels + 1);
+    p->w.assign(weights, weights + nlevels + 1);
+    p->dv = dv;
+    long total = 0;
+    for (int l = 0; l <= nlevels; l++) total += 4 * p->ops[l]->ng();
+    int st = p->pool.ensure(sizeof(double) * total, nullptr);
+    if (st != OK) {
+        delete p;
+        return st;
+    }
+    double* q = (double*)p->pool.p;
+    for (int l = 0; l <= nlevels; l++) {
+        const long ng = p->ops[l]->ng();
+        p->phi.push_back(q);
+        p->tmp.push_back(q + ng);
+        p->rho.push_back(q + 2 * ng);
+        p->res.push_back(q + 3 * ng);
+        q += 4 * ng;
+    }
+    *out = p;
+    return OK;
+}
+
+extern "C" void gfd_poisson_free(gfd_poisson* p)
+{
+    if (!p) return;
+    p->pool.release();
+    p->scalar.release();
+    delete p;
+}
+
+// gpaw/poisson.py:500-526 for one level; *cur/*alt as in relax_pp.
+static int vcycle(gfd_poisson* p, int l, double** cur, double** alt,
+                  const double* rho, hipStream_t s)
+{
+    gfd_operator* op = p->ops[l];
+    if (l < p->L) {
+        GFD_TRY(relax_pp(op, cur, alt, rho, p->pre[l], p->w[l], s));
+        EpiArgs e{};
+        e.aux = rho;
+        e.c0 = centre_coef(op);
+        GFD_TRY(op_pass(op, 0, *cur, p->res[l], 1, nullptr, EPI_RESID, e, s,
+                        nullptr));
+        XEpiArgs xe;
+        GFD_TRY(xfer(p->rst[l], p->res[l], p->rho[l + 1], 1, nullptr, xe, s));
+        double* c1 = p->phi[l + 1];
+        double* a1 = p->tmp[l + 1];
+        GFD_TRY(vec_zero(c1, p->ops[l + 1]->ng(), s));
+        GFD_TRY(vcycle(p, l + 1, &c1, &a1, p->rho[l + 1], s));
+        XEpiArgs ce;
+        ce.epi = XEPI_SUBFROM;
+        ce.base = *cur;
+        GFD_TRY(xfer(p->itp[l], c1, *alt, 1, nullptr, ce, s));
+        std::swap(*cur, *alt);
+    }
+    GFD_TRY(relax_pp(op, cur, alt, rho, p->post[l], p->w[l], s));
+    return OK;
+}
+
+static int poisson_iterate(gfd_poisson* p, double* phi, const double* rho,
+                           double* error, hipStream_t s)
+{
+    double* cur = phi;
+    double* alt = p->tmp[0];
+    GFD_TRY(vcycle(p, 0, &cur, &alt, rho, s));
+    if (cur != phi)
+        GFD_CHECK_HIP(hipMemcpyAsync(phi, cur, sizeof(double) * p->ops[0]->ng(),
+                                     hipMemcpyDeviceToDevice, s));
+    GFD_TRY(p->scalar.ensure(sizeof(double), s));
+    double* dn = (double*)p->scalar.p;
+    GFD_TRY(residual_norm(p->ops[0], 0, phi, rho, p->res[0], 1, nullptr, dn, s));
+    double h = 0.0;
+    GFD_CHECK_HIP(hipMemcpyAsync(&h, dn, sizeof(double), hipMemcpyDeviceToHost,
+                                 s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    *error = h * p->dv;
+    return OK;
+}
+
+extern "C" int gfd_poisson_iterate(gfd_poisson* p, double* phi,
+                                   const double* rho, double* error,
+                                   void* stream)
+{
+    if (!p || !phi || !rho || !error) {
+        set_error("gfd_poisson_iterate: bad arguments");
+        return EINVAL_;
+    }
+    return poisson_iterate(p, phi, rho, error, (hipStream_t)stream);
+}
+
+extern "C" int gfd_add_scalar(double* a, long n, double v, void* stream);
+
+extern "C" int gfd_poisson_solve(gfd_poisson* p, double* phi,
+                                 const double* rho, double eps, int maxiter,
+                                 int remove_mean, int* niter, double* error,
+                                 void* stream)
+{
+    if (!p || !phi || !rho || !niter || maxiter < 1) {
+        set_error("gfd_poisson_solve: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int it = 1;
+    double err = 0.0;
+    GFD_TRY(poisson_iterate(p, phi, rho, &err, s));
+    while (err > eps && it < maxiter) {
+        it++;
+        GFD_TRY(poisson_iterate(p, phi, rho, &err, s));
+    }
+    *niter = it;
+    if (error) *error = err;
+    if (it == maxiter) {
+        set_error("Poisson solver did not converge in %d iterations!",
+                  maxiter);
+        return ECONVERGE;
+    }
+    if (remove_mean) {
+        const long ng = p->ops[0]->ng();
+        double sum = 0.0;
+        GFD_TRY(p->ops[0]->partial.ensure(
+            sizeof(double) * (vec_dot_partials(ng) + 1), s));
+        double* dn = (double*)p->scalar.p;
+        GFD_TRY(vec_dot(phi, nullptr, ng, (double*)p->ops[0]->partial.p, dn, s));
+        GFD_CHECK_HIP(hipMemcpyAsync(&sum, dn, sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        GFD_CHECK_HIP(hipStreamSynchronize(s));
+        const double ave = sum / (double)ng;
+        GFD_TRY(gfd_add_scalar(phi, ng, -ave, stream));
+    }
+    return OK;
+}
+
+// ==================================================================== misc
+
+static DevBuf g_dot_buf;
+
+extern "C" int gfd_dot(const double* a, const double* b, long n,
+                       double* result, void* stream)
+{
+    if (!a || !result || n < 0) {
+        set_error("gfd_dot: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    GFD_TRY(g_dot_buf.ensure(sizeof(double) * (vec_dot_partials(n) + 2), s));
+    double* part = (double*)g_dot_buf.p;
+    double* dres = part + vec_dot_partials(n) + 1;
+    GFD_TRY(vec_dot(a, b, n, part, dres, s));
+    GFD_CHECK_HIP(hipMemcpyAsync(result, dres, sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+extern "C" int gfd_add_scalar(double* a, long n, double v, void* stream)
+{
+    if (!a || n < 0) {
+        set_error("gfd_add_scalar: bad arguments");
+        return EINVAL_;
+    }
+    return vec_add_scalar(a, n, v, (hipStream_t)stream);
+}
+
+extern "C" int gfd_set_tuning(const char* key, long value)
+{
+    if (!key) {
+        set_error("gfd_set_tuning: null key");
+        return EINVAL_;
+    }
+    if (std::strcmp(key, "force_general") == 0) {
+        g_force_general = value != 0;
+        return OK;
+    }
+    if (std::strcmp(key, "xm_tile") == 0) {
+        if (value == 0) {
+            xm_set_override(nullptr);
+            return OK;
+        }
+        XmTile t;
+        t.tz = (int)(value & 0xfff);
+        t.nseg = (int)((value >> 12) & 0xfff);
+        t.py = (int)((value >> 24) & 0xff);
+        t.xchunk = (int)((value >> 32) & 0xffff);
+        if (t.tz <= 0 || t.nseg <= 0 || t.py <= 0 || t.xchunk <= 0) {
+            set_error("gfd_set_tuning: bad xm_tile");
+            return EINVAL_;
+        }
+        xm_set_override(&t);
+        return OK;
+    }
+    set_error("gfd_set_tuning: unknown key '%s'", key);
+    return EINVAL_;
+}

@@ -1,0 +1,152 @@
+// Shared definitions for the gfx950 FD grid-path library.
+//
+// Data layout (HBM): every grid is C-order (n0, n1, n2) with n2 contiguous;
+// batches of grids are stored back to back (band-major), complex values are
+// interleaved (re, im) doubles (hip double2).  No padded copies are ever
+// made: the boundary is applied by index wrap in the loaders, with the
+// Bloch factor phase[axis][0] above the upper end and phase[axis][1] below
+// the lower end of a periodic axis (c/bc.cpp:242-257 semantics), and zero
+// outside a zero-boundary axis.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace gfd {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+enum Status : int {
+    OK = 0,
+    EINVAL_ = 1,    // bad argument
+    EHIP = 2,       // HIP runtime error
+    ENOMEM_ = 3,    // device allocation failed
+    ENOTSUP_ = 4,   // combination not supported
+    ECONVERGE = 5,  // iterative solver did not converge
+};
+
+#define GFD_CHECK_HIP(expr)                                                  \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            ::gfd::set_error("%s failed: %s (%s:%d)", #expr,                 \
+                             hipGetErrorString(e_), __FILE__, __LINE__);     \
+            return ::gfd::EHIP;                                              \
+        }                                                                    \
+    } while (0)
+
+#define GFD_LAUNCH_CHECK()                                                   \
+    do {                                                                     \
+        hipError_t e_ = hipGetLastError();                                   \
+        if (e_ != hipSuccess) {                                              \
+            ::gfd::set_error("kernel launch failed: %s (%s:%d)",             \
+                             hipGetErrorString(e_), __FILE__, __LINE__);     \
+            return ::gfd::EHIP;                                              \
+        }                                                                    \
+    } while (0)
+
+#define GFD_TRY(expr)                                                        \
+    do {                                                                     \
+        int s_ = (expr);                                                     \
+        if (s_ != ::gfd::OK) return s_;                                      \
+    } while (0)
+
+// ---------------------------------------------------------------- scalars
+typedef double2 cplx;
+
+__host__ __device__ inline double2 operator+(double2 a, double2 b)
+{
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__host__ __device__ inline double2 operator-(double2 a, double2 b)
+{
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+__host__ __device__ inline double2 operator*(double s, double2 a)
+{
+    return make_double2(s * a.x, s * a.y);
+}
+__host__ __device__ inline double2 operator-(double2 a)
+{
+    return make_double2(-a.x, -a.y);
+}
+__host__ __device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Scalar-generic helpers used by the templated kernels (T = double|double2
+// for the FP64 paths, float|float2 for the mixed-precision smoothers).
+template <typename T> struct ScalarOf;
+template <> struct ScalarOf<double> { typedef double type; };
+template <> struct ScalarOf<double2> { typedef double type; };
+template <> struct ScalarOf<float> { typedef float type; };
+
+template <typename T>
+__device__ inline T zero_of();
+template <> __device__ inline double zero_of<double>() { return 0.0; }
+template <> __device__ inline float zero_of<float>() { return 0.0f; }
+template <> __device__ inline double2 zero_of<double2>()
+{
+    return make_double2(0.0, 0.0);
+}
+
+// acc + c * v with c real
+__device__ inline double fmac(double acc, double c, double v)
+{
+    return fma(c, v, acc);
+}
+__device__ inline float fmac(float acc, float c, float v)
+{
+    return fmaf(c, v, acc);
+}
+__device__ inline double2 fmac(double2 acc, double c, double2 v)
+{
+    return make_double2(fma(c, v.x, acc.x), fma(c, v.y, acc.y));
+}
+
+// value times a boundary factor (1, 0 or a Bloch phase)
+__device__ inline double apply_phase(double v, double2 ph) { return ph.x * v; }
+__device__ inline float apply_phase(float v, double2 ph)
+{
+    return (float)ph.x * v;
+}
+__device__ inline double2 apply_phase(double2 v, double2 ph)
+{
+    return cmul(ph, v);
+}
+
+// ---------------------------------------------------------------- grids
+// Boundary of one grid: per axis a periodic flag and the two Bloch phases.
+struct BC {
+    int periodic[3];
+    double2 phase[3][2];  // [axis][0]: above the upper end, [1]: below lower
+};
+
+// Map a possibly out-of-range index along one axis to (index, factor).
+// Returns false when the value is zero (zero boundary).  `i` must lie in
+// [-n, 2n) -- the stencil range never exceeds the grid size.
+__device__ inline bool wrap_axis(int& i, int n, int periodic, double2 ph_hi,
+                                 double2 ph_lo, double2& f, bool& phased)
+{
+    if (i >= 0 && i < n) return true;
+    if (!periodic) return false;
+    if (i < 0) {
+        i += n;
+        f = phased ? cmul(ph_lo, f) : ph_lo;
+    } else {
+        i -= n;
+        f = phased ? cmul(ph_hi, f) : ph_hi;
+    }
+    phased = true;
+    return true;
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace gfd

@@ -1,0 +1,78 @@
+// Stencil kernel arguments and launchers (FDOperator.apply / relax and the
+// fused residual forms used by the multigrid drivers).
+#pragma once
+
+#include "gfd_common.h"
+
+namespace gfd {
+
+constexpr int MAXR_FAST = 4;   // largest radius of the x-marching kernel
+constexpr int MAXP = 256;      // largest number of stencil points
+
+// What the kernel does with acc = sum_{p != centre} c_p in(x + o_p) and the
+// centre value v = in(x):
+enum Epi : int {
+    EPI_PLAIN = 0,   // out = c0 v + acc
+    EPI_RELAX = 1,   // out = (1 - w) v + w (aux - acc) / c0        (Jacobi)
+    EPI_RESID = 2,   // out = (c0 v + acc) - aux          [+ sum out^2]
+    EPI_UPDATE = 3,  // out = +-(v - s ((c0 v + acc) - aux))
+};
+
+struct EpiArgs {
+    const void* aux;   // src (RELAX) or subtrahend (RESID/UPDATE), may be 0
+    double c0;
+    double w;          // RELAX weight
+    double s;          // UPDATE step
+    int negate;        // UPDATE: store -(...)
+    double* partial;   // RESID: per-block sum of |out|^2, or 0
+};
+
+// Axis-only (central-difference) stencil in per-axis form.
+struct AxisStencil {
+    int R;                    // radius (1..MAXR_FAST)
+    double c0;                // centre coefficient
+    double cp[3][MAXR_FAST];  // [axis][k-1]: coefficient of offset +k
+    double cm[3][MAXR_FAST];  // [axis][k-1]: coefficient of offset -k
+    int has[3];               // axis carries nonzero coefficients
+};
+
+// General stencil: coefficients and offsets live in device memory.
+struct GeneralStencil {
+    int P;
+    int centre;               // index of the (first) zero offset, -1 if none
+    double c0;
+    const double* coef;       // [P] (centre term excluded from the sum)
+    const int* off;           // [P][3]
+};
+
+// Launch one stencil pass over `nin` grids of shape n[3].
+// dtype: 0 = f64, 1 = c128, 2 = f32.
+int launch_axis_stencil(const AxisStencil& st, int dtype, const int n[3],
+                        const BC& bc, const void* in, void* out, long nin,
+                        int epi, const EpiArgs& ea, hipStream_t stream,
+                        int* nblocks_out);
+int launch_general_stencil(const GeneralStencil& st, int dtype,
+                           const int n[3], const BC& bc, const void* in,
+                           void* out, long nin, int epi, const EpiArgs& ea,
+                           hipStream_t stream, int* nblocks_out);
+
+// Block-count upper bounds for the residual-norm partial buffer.
+long axis_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
+                             long nin);
+long general_stencil_max_blocks(const int n[3], long nin);
+
+// Deterministic final reduction of `np` partial sums into *result (device).
+int launch_sum_partials(const double* partial, long np, double* result,
+                        hipStream_t stream);
+
+// Tile choice of the x-marching kernel (exposed for the autotuner).
+struct XmTile {
+    int tz;      // tile width along z (threads per row)
+    int nseg;    // row segments (threads along y)
+    int py;      // rows per thread
+    int xchunk;  // planes per block along x
+};
+XmTile xm_default_tile(int R, int dtype, const int n[3], long nin);
+void xm_set_override(const XmTile* t);  // nullptr clears
+
+}  // namespace gfd

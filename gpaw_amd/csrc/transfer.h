@@ -1,0 +1,48 @@
+// Restriction / interpolation (Transformer.apply) on gfx950.
+#pragma once
+
+#include "gfd_common.h"
+
+namespace gfd {
+
+// What the last pass does with the transferred value T:
+enum XEpi : int {
+    XEPI_PLAIN = 0,  // out = T
+    XEPI_NEG = 1,    // out = -T
+    XEPI_SCALE2 = 2, // out = T, out2 = alpha T          (alpha negated w/ neg)
+    XEPI_SUBFROM = 3 // out = base - T
+};
+
+struct XEpiArgs {
+    int epi = XEPI_PLAIN;
+    int negate = 0;          // applies to T before the epilogue (SCALE2)
+    double alpha = 0.0;
+    void* out2 = nullptr;
+    const void* base = nullptr;
+};
+
+// Per-axis description of one 1-D transfer pass.
+struct XferAxis {
+    int nin;     // input points along the axis
+    int nout;    // output points along the axis
+    int off;     // interior index of the first centre (see transfer.hip)
+    int skip0;   // interpolation: first even output dropped
+};
+
+struct XferPlan {
+    int interpolate;
+    int K;           // 2, 4, 6, 8
+    int nin[3];      // input grid
+    int nout[3];     // output grid
+    XferAxis ax[3];
+};
+
+// Scratch (in elements of the data type) needed for `nin` grids.
+long xfer_scratch_elems(const XferPlan& p, long nin);
+
+// dtype: 0 f64, 1 c128, 2 f32
+int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
+                    const void* in, void* out, long nin, void* scratch,
+                    const XEpiArgs& ea, hipStream_t stream);
+
+}  // namespace gfd

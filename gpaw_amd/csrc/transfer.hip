@@ -1,0 +1,266 @@
+// Multigrid transfer kernels for gfx950: K-tap restriction and
+// interpolation (K = 2 nn = 2, 4, 6, 8), real and complex, periodic / zero /
+// Bloch-phase boundaries.
+//
+// A 3-D transfer is the product of three 1-D transfers (z, then y, then x,
+// the order of c/bmgs/restrict.cpp:153-155 and interpolate.cpp:245-256).
+// Each pass is one kernel with one thread per output point; the boundary of
+// the pass's axis is taken by index wrap, the other two axes are in range.
+//
+// 1-D semantics in interior-index form (derived from the reference's padded
+// buffers, c/bmgs/restrict.cpp:18-63 and interpolate.cpp:10-46):
+//   restrict:    out[i] = 1/2 (a[c] + sum_j w_j (a[c+j] + a[c-j])),
+//                c = 2 i + off, j = 1, 3, .., K-1
+//   interpolate: with p = (i + skip0) / 2, c = p + off:
+//                even (i + skip0): out[i] = a[c]
+//                odd:              out[i] = sum_j w_j (a[c-j] + a[c+1+j]),
+//                j = 0 .. K/2-1
+// The weights are exact dyadic rationals.
+#include "transfer.h"
+
+namespace gfd {
+
+template <int K> struct W;
+template <> struct W<2> {
+    static constexpr double w[1] = {0.5};
+};
+template <> struct W<4> {
+    static constexpr double w[2] = {0.5625, -0.0625};
+};
+template <> struct W<6> {
+    static constexpr double w[3] = {0.58593750, -0.09765625, 0.01171875};
+};
+template <> struct W<8> {
+    static constexpr double w[4] = {0.59814453125, -0.11962890625,
+                                    0.02392578125, -0.00244140625};
+};
+
+template <typename T> struct XIsCplx { static constexpr bool value = false; };
+template <> struct XIsCplx<double2> { static constexpr bool value = true; };
+
+template <typename T>
+struct XArgs {
+    const T* in;
+    T* out;
+    T* out2;
+    const T* base;
+    int d[3];      // output dims of this pass
+    int ilen;      // input length along the axis
+    long istride;  // input stride along the axis
+    long in_grid;  // input points per grid
+    long out_grid; // output points per grid
+    long total;
+    int off, skip0;
+    int periodic;
+    double2 ph_hi, ph_lo;
+    int negate;
+    double alpha;
+};
+
+template <typename T>
+__device__ inline T xload(const XArgs<T>& a, const T* __restrict__ g,
+                          long pos, int j)
+{
+    if (j >= 0 && j < a.ilen) return g[pos + (long)j * a.istride];
+    if (!a.periodic) return zero_of<T>();
+    double2 f = make_double2(1.0, 0.0);
+    bool first = true;
+    while (j < 0) {
+        j += a.ilen;
+        f = first ? a.ph_lo : cmul(a.ph_lo, f);
+        first = false;
+    }
+    while (j >= a.ilen) {
+        j -= a.ilen;
+        f = first ? a.ph_hi : cmul(a.ph_hi, f);
+        first = false;
+    }
+    T v = g[pos + (long)j * a.istride];
+    if constexpr (XIsCplx<T>::value) v = cmul(f, v);
+    return v;
+}
+
+template <typename T, int AX, int K, bool INTERP, int EPI>
+__global__ void __launch_bounds__(256) k_xfer1d(XArgs<T> a)
+{
+    typedef typename ScalarOf<T>::type S;
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.total) return;
+    const long b = gid / a.out_grid;
+    long r = gid - b * a.out_grid;
+    const int o2 = (int)(r % a.d[2]);
+    r /= a.d[2];
+    const int o1 = (int)(r % a.d[1]);
+    const int o0 = (int)(r / a.d[1]);
+    // input position with the pass axis at 0
+    long pos;
+    int i;
+    if (AX == 2) {
+        pos = ((long)o0 * a.d[1] + o1) * a.ilen;
+        i = o2;
+    } else if (AX == 1) {
+        pos = (long)o0 * a.ilen * a.d[2] + o2;
+        i = o1;
+    } else {
+        pos = (long)o1 * a.d[2] + o2;
+        i = o0;
+    }
+    const T* __restrict__ g = a.in + b * a.in_grid;
+    T v;
+    if constexpr (!INTERP) {
+        const int c = 2 * i + a.off;
+        T acc = xload(a, g, pos, c);
+#pragma unroll
+        for (int m = 0; m < K / 2; m++) {
+            const int j = 2 * m + 1;
+            T pair = xload(a, g, pos, c + j) + xload(a, g, pos, c - j);
+            acc = fmac(acc, (S)W<K>::w[m], pair);
+        }
+        v = (S)0.5 * acc;
+    } else {
+        const int q = i + a.skip0;
+        const int c = (q >> 1) + a.off;
+        if ((q & 1) == 0) {
+            v = xload(a, g, pos, c);
+        } else {
+            T pair0 = xload(a, g, pos, c) + xload(a, g, pos, c + 1);
+            T acc = (S)W<K>::w[0] * pair0;
+#pragma unroll
+            for (int m = 1; m < K / 2; m++) {
+                T pair = xload(a, g, pos, c - m) + xload(a, g, pos, c + 1 + m);
+                acc = fmac(acc, (S)W<K>::w[m], pair);
+            }
+            v = acc;
+        }
+    }
+    if constexpr (EPI == XEPI_PLAIN) {
+        a.out[gid] = v;
+    } else if constexpr (EPI == XEPI_NEG) {
+        a.out[gid] = -v;
+    } else if constexpr (EPI == XEPI_SCALE2) {
+        if (a.negate) v = -v;
+        a.out[gid] = v;
+        a.out2[gid] = (S)a.alpha * v;
+    } else {
+        a.out[gid] = a.base[gid] - v;
+    }
+}
+
+long xfer_scratch_elems(const XferPlan& p, long nin)
+{
+    // pass z output + pass y output
+    const long t1 = (long)p.nin[0] * p.nin[1] * p.nout[2];
+    const long t2 = (long)p.nin[0] * p.nout[1] * p.nout[2];
+    return (t1 + t2) * nin;
+}
+
+template <typename T, int AX, int K, bool INTERP, int EPI>
+static int launch_pass(XArgs<T> a, hipStream_t s)
+{
+    const long nb = (a.total + 255) / 256;
+    if (nb <= 0) return OK;
+    if (nb > 0x7fffffffL) {
+        set_error("transfer: too many blocks");
+        return ENOTSUP_;
+    }
+    hipLaunchKernelGGL((k_xfer1d<T, AX, K, INTERP, EPI>), dim3((unsigned)nb),
+                       dim3(256), 0, s, a);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T, int AX, int K, bool INTERP>
+static int launch_pass_epi(XArgs<T> a, int epi, hipStream_t s)
+{
+    switch (epi) {
+    case XEPI_PLAIN: return launch_pass<T, AX, K, INTERP, XEPI_PLAIN>(a, s);
+    case XEPI_NEG: return launch_pass<T, AX, K, INTERP, XEPI_NEG>(a, s);
+    case XEPI_SCALE2: return launch_pass<T, AX, K, INTERP, XEPI_SCALE2>(a, s);
+    case XEPI_SUBFROM: return launch_pass<T, AX, K, INTERP, XEPI_SUBFROM>(a, s);
+    }
+    set_error("transfer: bad epilogue %d", epi);
+    return EINVAL_;
+}
+
+template <typename T, int K, bool INTERP>
+static int run_transfer(const XferPlan& p, const BC& bc, const void* in,
+                        void* out, long nin, void* scratch,
+                        const XEpiArgs& ea, hipStream_t s)
+{
+    T* t1 = (T*)scratch;
+    T* t2 = t1 + (long)p.nin[0] * p.nin[1] * p.nout[2] * nin;
+    const T* src = (const T*)in;
+    int cur[3] = {p.nin[0], p.nin[1], p.nin[2]};
+    // z, y, x
+    for (int step = 0; step < 3; step++) {
+        const int ax = 2 - step;
+        XArgs<T> a;
+        a.in = src;
+        a.ilen = cur[ax];
+        int d[3] = {cur[0], cur[1], cur[2]};
+        d[ax] = p.nout[ax];
+        a.d[0] = d[0];
+        a.d[1] = d[1];
+        a.d[2] = d[2];
+        a.istride = ax == 2 ? 1 : (ax == 1 ? (long)cur[2] : (long)cur[1] * cur[2]);
+        a.in_grid = (long)cur[0] * cur[1] * cur[2];
+        a.out_grid = (long)d[0] * d[1] * d[2];
+        a.total = a.out_grid * nin;
+        a.off = p.ax[ax].off;
+        a.skip0 = p.ax[ax].skip0;
+        a.periodic = bc.periodic[ax];
+        a.ph_hi = bc.phase[ax][0];
+        a.ph_lo = bc.phase[ax][1];
+        a.negate = ea.negate;
+        a.alpha = ea.alpha;
+        a.out2 = (T*)ea.out2;
+        a.base = (const T*)ea.base;
+        const bool last = (step == 2);
+        a.out = last ? (T*)out : (step == 0 ? t1 : t2);
+        const int epi = last ? ea.epi : XEPI_PLAIN;
+        int st;
+        if (ax == 2) st = launch_pass_epi<T, 2, K, INTERP>(a, epi, s);
+        else if (ax == 1) st = launch_pass_epi<T, 1, K, INTERP>(a, epi, s);
+        else st = launch_pass_epi<T, 0, K, INTERP>(a, epi, s);
+        if (st != OK) return st;
+        src = a.out;
+        cur[ax] = p.nout[ax];
+    }
+    return OK;
+}
+
+template <typename T>
+static int run_transfer_k(const XferPlan& p, const BC& bc, const void* in,
+                          void* out, long nin, void* scratch,
+                          const XEpiArgs& ea, hipStream_t s)
+{
+    const bool I = p.interpolate != 0;
+    switch (p.K) {
+    case 2: return I ? run_transfer<T, 2, true>(p, bc, in, out, nin, scratch, ea, s)
+                     : run_transfer<T, 2, false>(p, bc, in, out, nin, scratch, ea, s);
+    case 4: return I ? run_transfer<T, 4, true>(p, bc, in, out, nin, scratch, ea, s)
+                     : run_transfer<T, 4, false>(p, bc, in, out, nin, scratch, ea, s);
+    case 6: return I ? run_transfer<T, 6, true>(p, bc, in, out, nin, scratch, ea, s)
+                     : run_transfer<T, 6, false>(p, bc, in, out, nin, scratch, ea, s);
+    case 8: return I ? run_transfer<T, 8, true>(p, bc, in, out, nin, scratch, ea, s)
+                     : run_transfer<T, 8, false>(p, bc, in, out, nin, scratch, ea, s);
+    }
+    set_error("transfer: unsupported K=%d", p.K);
+    return EINVAL_;
+}
+
+int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
+                    const void* in, void* out, long nin, void* scratch,
+                    const XEpiArgs& ea, hipStream_t stream)
+{
+    if (dtype == 0)
+        return run_transfer_k<double>(p, bc, in, out, nin, scratch, ea, stream);
+    if (dtype == 1)
+        return run_transfer_k<double2>(p, bc, in, out, nin, scratch, ea, stream);
+    if (dtype == 2)
+        return run_transfer_k<float>(p, bc, in, out, nin, scratch, ea, stream);
+    set_error("transfer: bad dtype %d", dtype);
+    return EINVAL_;
+}
+
+}  // namespace gfd

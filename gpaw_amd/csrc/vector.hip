@@ -1,0 +1,98 @@
+// Element-wise and reduction kernels (grid-stride, 16-byte accesses where
+// the data allows) used by the drivers.
+#include "vector.h"
+
+namespace gfd {
+
+__global__ void __launch_bounds__(256)
+k_fill_zero(double* __restrict__ a, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_copy(const double* __restrict__ a, double* __restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        b[i] = a[i];
+}
+
+__global__ void __launch_bounds__(256)
+k_add_scalar(double* __restrict__ a, long n, double v)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i] += v;
+}
+
+// per-block partial sums of a_i * b_i (b == nullptr: sum of a_i)
+__global__ void __launch_bounds__(256)
+k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
+              long n, double* __restrict__ partial)
+{
+    double s = 0.0;
+    if (b) {
+        for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+             i += (long)gridDim.x * 256)
+            s = fma(a[i], b[i], s);
+    } else {
+        for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+             i += (long)gridDim.x * 256)
+            s += a[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+static unsigned grid_for(long n)
+{
+    long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+int vec_zero(double* a, long n, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n)), dim3(256), 0, s, a, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_copy(const double* a, double* b, long n, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_add_scalar(double* a, long n, double v, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_add_scalar, dim3(grid_for(n)), dim3(256), 0, s, a,
+                       n, v);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+long vec_dot_partials(long n) { return n <= 0 ? 1 : grid_for(n); }
+
+int vec_dot(const double* a, const double* b, long n, double* partial,
+            double* dev_result, hipStream_t s)
+{
+    const unsigned g = n <= 0 ? 1 : grid_for(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(256), 0, s, a, b,
+                       n < 0 ? 0 : n, partial);
+    GFD_LAUNCH_CHECK();
+    return launch_sum_partials(partial, g, dev_result, s);
+}
+
+}  // namespace gfd

@@ -1,0 +1,106 @@
+"""Multigrid kinetic preconditioner on the GPU (drop-in for
+gpaw.preconditioner.Preconditioner, gpaw/preconditioner.py:9-93).
+
+The whole 3-level chain of c/fd_preconditioner.cpp:221-345 runs as one
+native call (gfd_fd_precond) over a block of bands: four transfers and
+three stencil passes with the element-wise steps fused into them.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .arrays import Staging, device, is_device, ptr, stream_handle, torch
+from .fd_operators import Laplace
+from .transformers import Transformer
+
+
+class Preconditioner:
+    def __init__(self, gd0, kin0, dtype=float, block=1, xp=np):
+        gd1 = gd0.coarsen()
+        gd2 = gd1.coarsen()
+        self.gd0, self.gd1, self.gd2 = gd0, gd1, gd2
+        self.kin0 = kin0
+        self.kin1 = Laplace(gd1, -0.5, 1, dtype, xp=xp)
+        self.kin2 = Laplace(gd2, -0.5, 1, dtype, xp=xp)
+        self.dtype = dtype
+        self.block = block
+        self.step = 0.66666666 / kin0.get_diagonal_element()
+        self.restrictor_object0 = Transformer(gd0, gd1, 1, dtype, xp=xp)
+        self.restrictor_object1 = Transformer(gd1, gd2, 1, dtype, xp=xp)
+        self.interpolator_object2 = Transformer(gd2, gd1, 1, dtype, xp=xp)
+        self.interpolator_object1 = Transformer(gd1, gd0, 1, dtype, xp=xp)
+        self.restrictor0 = self.restrictor_object0.apply
+        self.restrictor1 = self.restrictor_object1.apply
+        self.interpolator2 = self.interpolator_object2.apply
+        self.interpolator1 = self.interpolator_object1.apply
+        self.use_c_precond = True
+        self.xp = xp
+        self._scratch = None
+
+    def calculate_kinetic_energy(self, psit_xG, kpt):
+        return None
+
+    def _work(self, nb):
+        if self._scratch is None or self._scratch[0].shape[1] < nb:
+            tdt = torch.complex128 if self.dtype == complex else torch.float64
+            dev = device()
+            n = max(nb, self.block)
+            self._scratch = (
+                torch.zeros((2, n) + tuple(self.gd0.n_c), dtype=tdt,
+                            device=dev),
+                torch.zeros((3, n) + tuple(self.gd1.n_c), dtype=tdt,
+                            device=dev),
+                torch.zeros((3, n) + tuple(self.gd2.n_c), dtype=tdt,
+                            device=dev))
+        return self._scratch
+
+    def __call__(self, residuals, kpt, ekin=None, out=None):
+        if residuals.ndim == 3:
+            if out is None:
+                return self.__call__(residuals[None], kpt)[0]
+            return self.__call__(residuals[None], kpt, out=out[None])[0]
+        nb = len(residuals)
+        if nb > self.block:
+            if out is None:
+                out = (torch.empty_like(residuals) if is_device(residuals)
+                       else np.empty_like(residuals))
+            for n1 in range(0, nb, self.block):
+                self(residuals[n1:n1 + self.block], kpt,
+                     out=out[n1:n1 + self.block])
+            return out
+        if tuple(residuals.shape[-3:]) != tuple(self.gd0.n_c):
+            raise TypeError('Input grid shape does not match with arrays d0 '
+                            'and q0.')
+        phases = getattr(kpt, 'phase_cd', None) if kpt is not None else None
+        s0, s1, s2 = self._work(nb)
+        q0 = s0[0, :nb]
+        r1, d1, q1 = s1[0, :nb], s1[1, :nb], s1[2, :nb]
+        r2, d2, q2 = s2[0, :nb], s2[1, :nb], s2[2, :nb]
+        st = Staging(residuals)
+        res = st.dev[0].contiguous()
+        if out is None:
+            d0 = s0[1, :nb] if not is_device(residuals) else \
+                torch.empty_like(res)
+        else:
+            d0 = out if is_device(out) else torch.empty_like(res)
+        keep, ph = (None, None)
+        if self.dtype == complex:
+            keep, ph = _lib.phases_arg(np.ones((3, 2), complex)
+                                       if phases is None else phases)
+        lib = _lib.load()
+        _lib.check(lib.gfd_fd_precond(
+            self.restrictor_object0._handle(),
+            self.restrictor_object1._handle(),
+            self.interpolator_object1._handle(),
+            self.interpolator_object2._handle(),
+            self.kin0._handle(), self.kin1._handle(), self.kin2._handle(),
+            ptr(d0), ptr(q0), ptr(r1), ptr(d1), ptr(q1), ptr(r2), ptr(d2),
+            ptr(q2), ptr(res), nb, float(self.step), ph, stream_handle()),
+            'fd_precond')
+        if out is not None and not is_device(out):
+            out[...] = d0.cpu().numpy()
+            return out
+        if not is_device(residuals):
+            return d0.cpu().numpy()
+        return d0

@@ -1,0 +1,157 @@
+/* C ABI of the gfx950 finite-difference grid-path library (libgpaw_fd.so).
+ *
+ * Drop-in boundary for the GPAW FD grid hot path.  Every entry point below
+ * replaces one binding of the reference's C extension `_gpaw` (method table
+ * c/_gpaw.h:182-194 and the GPU methods in c/gpu/operators.cpp,
+ * c/gpu/transformers.cpp); each declaration cites the function it replaces.
+ *
+ * Conventions
+ *   - Plain C types only.  Array arguments are device pointers (HBM) to
+ *     C-order grids (n0, n1, n2), n2 contiguous, `nin` grids back to back;
+ *     complex data is interleaved (re, im) double.
+ *   - `phases` is the reference's phase_cd, a (3, 2) complex array given as
+ *     12 doubles [axis][side][re, im]; NULL for real data.
+ *   - `stream` is a hipStream_t (NULL: the null stream).  Calls are
+ *     asynchronous on that stream unless noted; the caller synchronises.
+ *   - Every function returns a status (GFD_OK == 0); on failure
+ *     gfd_last_error() describes it.  The library never aborts the process.
+ *   - Handles own device memory for their scratch; one handle must not be
+ *     used from two streams at once.
+ */
+#ifndef GPAW_FD_H
+#define GPAW_FD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GFD_OK = 0,
+    GFD_EINVAL = 1,    /* bad argument (reference: TypeError / assert) */
+    GFD_EHIP = 2,      /* HIP runtime error */
+    GFD_ENOMEM = 3,    /* device allocation failed */
+    GFD_ENOTSUP = 4,   /* combination not supported on the GPU */
+    GFD_ECONVERGE = 5  /* reference: PoissonConvergenceError */
+};
+
+/* Message for the last failing call on this thread. */
+const char* gfd_last_error(void);
+
+/* ABI version of this header. */
+int gfd_abi_version(void);
+
+typedef struct gfd_operator gfd_operator;
+typedef struct gfd_transformer gfd_transformer;
+typedef struct gfd_poisson gfd_poisson;
+
+/* ---------------------------------------------------------------- stencils
+ * Replaces _gpaw.Operator(coefs, offsets, size, range, neighbors, real,
+ * comm, cfd[, use_gpu]) -- c/operators.cpp:379-420, bound from
+ * gpaw/fd_operators.py:91-93.  `offsets` are the flattened offsets with the
+ * padded strides ((n1+2r)(n2+2r), n2+2r, 1) of gpaw/fd_operators.py:54-63;
+ * neighbors is neighbor_cd (3, 2): >= 0 periodic (single domain), -1 zero.
+ * comm must be serial (single GPU); cfd is recomputed from the offsets. */
+int gfd_operator_new(int ncoefs, const double* coefs, const long* offsets,
+                     const long size[3], int range, const long neighbors[6],
+                     int real, int cfd, gfd_operator** out);
+void gfd_operator_free(gfd_operator* op);
+
+/* Replaces Operator.apply_gpu(in, out, shape, dtype[, phases]) --
+ * c/gpu/operators.cpp:344-385 (CPU: Operator.apply, c/operators.cpp:245-316).
+ * nin = number of grids.  in and out must not overlap. */
+int gfd_operator_apply(gfd_operator* op, const void* in, void* out, long nin,
+                       const double* phases, void* stream);
+
+/* Replaces Operator.relax_gpu(method, f, src, nrelax[, w]) --
+ * c/gpu/operators.cpp:200-221 (CPU: Operator.relax, c/operators.cpp:50-92).
+ * method 2 = weighted Jacobi; method 1 (Gauss-Seidel) returns GFD_ENOTSUP,
+ * as the reference GPU kernel does not implement it
+ * (c/gpu/kernels/relax.cpp:72-80).  Real data only. */
+int gfd_operator_relax(gfd_operator* op, int method, double* f,
+                       const double* src, int nrelax, double w, void* stream);
+
+/* Replaces Operator.get_diagonal_element() -- c/operators.cpp:319-332. */
+double gfd_operator_get_diagonal_element(const gfd_operator* op);
+
+/* Fused residual r = op(in) - b with sum |r|^2 written to *norm2 (host,
+ * synchronous when norm2 != NULL).  Replaces the pair
+ * `op.apply(phi, r); r -= rho; dot(r, r)` of gpaw/poisson.py:527-532. */
+int gfd_operator_residual(gfd_operator* op, const void* in, const void* b,
+                          void* r, long nin, const double* phases,
+                          double* norm2, void* stream);
+
+/* ---------------------------------------------------------------- transfers
+ * Replaces _gpaw.Transformer(size_in, size_out, k, pad, npad, skip,
+ * neighbors, real, comm, interpolate[, use_gpu]) --
+ * c/transformers.cpp:196-246, bound from gpaw/transformers.py:71-77. */
+int gfd_transformer_new(const long size_in[3], const long size_out[3], int k,
+                        const long pad[6], const long npad[6],
+                        const long skip[6], const long neighbors[6], int real,
+                        int interpolate, gfd_transformer** out);
+void gfd_transformer_free(gfd_transformer* t);
+
+/* Replaces Transformer.apply_gpu(in, out, shape, dtype[, phases]) --
+ * c/gpu/transformers.cpp:189-220 (CPU: c/transformers.cpp:112-158). */
+int gfd_transformer_apply(gfd_transformer* t, const void* in, void* out,
+                          long nin, const double* phases, void* stream);
+
+/* ---------------------------------------------------------------- precond
+ * Replaces _gpaw.fd_precond(rest0, rest1, intp1, intp2, kin0, kin1, kin2,
+ * d0, q0, r1, d1, q1, r2, d2, q2, residuals, nresiduals, step[, phases]) --
+ * c/fd_preconditioner.cpp:106-351, bound from gpaw/preconditioner.py:68-72.
+ * The negated residuals are not needed (the sign is folded into the first
+ * restriction).  Result in d0; q0, r1, d1, q1, r2, d2, q2 are work arrays of
+ * nin grids each on the fine, first and second coarse levels. */
+int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
+                   gfd_transformer* intp1, gfd_transformer* intp2,
+                   gfd_operator* kin0, gfd_operator* kin1, gfd_operator* kin2,
+                   void* d0, void* q0, void* r1, void* d1, void* q1, void* r2,
+                   void* d2, void* q2, const void* residuals, long nin,
+                   double step, const double* phases, void* stream);
+
+/* ---------------------------------------------------------------- Poisson
+ * Native multigrid driver of FDPoissonSolver (gpaw/poisson.py:321-537):
+ * ops[0..nlevels] are the Laplacians of levels 0..L, restrictors[l] /
+ * interpolators[l] map level l <-> l+1.  Level buffers are allocated here
+ * (gpaw/poisson.py:451-472). */
+int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
+                    gfd_transformer* const* restrictors,
+                    gfd_transformer* const* interpolators,
+                    const int* presmooths, const int* postsmooths,
+                    const double* weights, int relax_method, double dv,
+                    gfd_poisson** out);
+void gfd_poisson_free(gfd_poisson* p);
+
+/* One V-cycle, FDPoissonSolver.iterate2(step) at level 0
+ * (gpaw/poisson.py:500-537); *error = dv sum r^2 (synchronous). */
+int gfd_poisson_iterate(gfd_poisson* p, double* phi, const double* rho,
+                        double* error, void* stream);
+
+/* The loop of FDPoissonSolver.solve_neutral (gpaw/poisson.py:474-498):
+ * iterate while error > eps and niter < maxiter; returns GFD_ECONVERGE when
+ * niter reaches maxiter.  remove_mean: subtract the average of phi
+ * (fully periodic grids, :492-496).  Synchronous. */
+int gfd_poisson_solve(gfd_poisson* p, double* phi, const double* rho,
+                      double eps, int maxiter, int remove_mean, int* niter,
+                      double* error, void* stream);
+
+/* ---------------------------------------------------------------- misc */
+
+/* *result = sum_i a_i b_i over n doubles (b == NULL: sum_i a_i); the
+ * numpy dot / sum of gpaw/poisson.py:493,531.  Synchronous. */
+int gfd_dot(const double* a, const double* b, long n, double* result,
+            void* stream);
+
+/* a_i += v for n doubles (the `phi -= phi_ave` of gpaw/poisson.py:496). */
+int gfd_add_scalar(double* a, long n, double v, void* stream);
+
+/* Tuning knobs (no reference counterpart): key "force_general" (0/1) routes
+ * every stencil through the general kernel; "xm_tile" with value
+ * tz | nseg << 12 | py << 24 | xchunk << 32 overrides the x-march tile
+ * (value 0 clears).  Returns GFD_EINVAL for unknown keys. */
+int gfd_set_tuning(const char* key, long value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPAW_FD_H */

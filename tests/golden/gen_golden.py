@@ -1,0 +1,332 @@
+"""Write the golden fixtures of the FD grid path from the reference code.
+
+Runs only where the reference sources are present (the build container):
+it loads ``oracle/_ref/_gpaw_ref*.so`` -- the reference's own C sources for
+this path (c/bmgs/*.cpp, c/bc.cpp, c/operators.cpp, c/transformers.cpp,
+c/fd_preconditioner.cpp) compiled by ``make -C oracle ref`` -- and drives its
+``Operator``, ``Transformer`` and ``fd_precond`` with the coefficient and
+padding glue of gpaw_amd (numpy only; pinned separately by closed forms).
+
+Fixtures (inputs are stored with the outputs):
+  fd_golden.npz        stencil apply / relax / transfer cases
+  precond_golden.npz   fd_precond (real periodic, complex Bloch, real zero)
+  poisson_golden.npz   V-cycle runs: 16^3 zero-boundary and the 48^3
+                       periodic 4-Gaussian problem (niter, error, phi)
+
+Usage: python tests/golden/gen_golden.py
+"""
+from __future__ import annotations
+
+import glob
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, '..', '..'))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from gpaw_amd import workloads  # noqa: E402
+from gpaw_amd.fd_operators import (GUCLaplace, Gradient, Laplace,  # noqa
+                                   LaplaceA)
+from gpaw_amd.grid_descriptor import GridDescriptor  # noqa: E402
+from gpaw_amd.transformers import Transformer  # noqa: E402
+
+
+def load_ref():
+    so = glob.glob(os.path.join(ROOT, 'oracle', '_ref', '_gpaw_ref*.so'))
+    if not so:
+        raise SystemExit('oracle/_ref is not built (make -C oracle ref)')
+    sys.path.insert(0, os.path.dirname(so[0]))
+    import _gpaw_ref
+    return _gpaw_ref
+
+
+G = load_ref()
+
+
+def ref_operator(spec, real):
+    r = int(np.abs(spec.offset_pc).max())
+    M = spec.n_c + 2 * r
+    stride = np.array([M[1] * M[2], M[2], 1])
+    off = np.ascontiguousarray(spec.offset_pc @ stride)
+    return G.Operator(spec.coef_p, off, np.ascontiguousarray(spec.n_c), r,
+                      np.ascontiguousarray(spec.neighbor_cd), int(real), None,
+                      int(oracle.is_cfd(spec.offset_pc)))
+
+
+def ref_transformer(t, real):
+    return G.Transformer(np.ascontiguousarray(t.size_in),
+                         np.ascontiguousarray(t.size_out), t.k,
+                         np.ascontiguousarray(t.pad_cd),
+                         np.ascontiguousarray(t.npad_cd),
+                         np.ascontiguousarray(t.skip_cd),
+                         np.ascontiguousarray(t.neighbor_cd), int(real), None,
+                         int(t.interpolate))
+
+
+class RefBackend:
+    """oracle-glue backend running the reference extension."""
+
+    def apply(self, op, a, out, phases=None):
+        real = a.dtype == np.float64
+        o = ref_operator(op, real)
+        if real:
+            o.apply(np.ascontiguousarray(a), out)
+        else:
+            o.apply(np.ascontiguousarray(a), out,
+                    np.ones((3, 2), complex) if phases is None else phases)
+
+    def relax(self, op, f, src, n, w):
+        ref_operator(op, True).relax(2, f, np.ascontiguousarray(src), n, w)
+
+    def transform(self, t, a, out, phases=None):
+        real = a.dtype == np.float64
+        T = ref_transformer(t, real)
+        a = np.ascontiguousarray(a)
+        lead = a.shape[:-3]
+        a2 = a.reshape((-1,) + tuple(t.size_in))
+        o2 = out.reshape((-1,) + tuple(t.size_out))
+        nout = int(np.prod(t.size_out))
+        # the reference's first interpolation pass is written into the
+        # output array and can exceed it (zero boundaries, small grids):
+        # give it room, one grid at a time
+        slack = int(np.prod(t.size_in + t.pad_cd.sum(1))) * 16
+        for i in range(len(a2)):
+            big = np.zeros(nout + slack, a.dtype)
+            view = big[:nout].reshape(tuple(t.size_out))
+            if real:
+                T.apply(a2[i], view)
+            else:
+                T.apply(a2[i], view,
+                        np.ones((3, 2), complex) if phases is None else phases)
+            o2[i] = view
+        assert out.shape == lead + tuple(t.size_out)
+
+
+def spec(op):
+    return oracle.StencilSpec(op.coef_p, op.offset_pc, op.gd.n_c,
+                              op.gd.neighbor_cd)
+
+
+def tspec(t):
+    return oracle.TransferSpec(t.interpolate, 2 * t.nn, t.gdin.n_c,
+                               t.gdout.n_c, t.pad_cd, t.neighborpad_cd,
+                               t.skip_cd, t.gdin.neighbor_cd)
+
+
+def rand(rng, shape, dtype):
+    a = rng.uniform(-1, 1, shape)
+    if dtype == complex:
+        a = a + 1j * rng.uniform(-1, 1, shape)
+    return a
+
+
+def bloch_phases(rng, pbc):
+    k = rng.uniform(-0.5, 0.5, 3) * pbc
+    return np.exp(2j * np.pi * k[:, None] * np.array([1, -1]))
+
+
+def fd_cases():
+    rng = np.random.default_rng(20261202)
+    out = {}
+    hexa = [(3.0, 0, 0), (1.5, 3**0.5 * 1.5, 0), (0, 0, 3.3)]
+    fcc = [(0, 2.0, 2.0), (2.0, 0, 2.0), (2.0, 2.0, 0)]
+    stencils = [
+        ('lap3_pbc_real', (12, 10, 14), (3.0, 3.3, 3.6), True, 'lap', 3,
+         float),
+        ('lap3_zero_real', (12, 10, 14), (3.0, 3.3, 3.6), False, 'lap', 3,
+         float),
+        ('lap2_mixed_cplx', (10, 12, 8), (3.0, 3.0, 2.4),
+         (True, False, True), 'lap', 2, complex),
+        ('lap1_pbc_cplx', (8, 6, 10), (2.0, 2.0, 2.0), True, 'lap', 1,
+         complex),
+        ('lap4_pbc_real', (10, 10, 12), (2.0, 2.0, 2.0), True, 'lap', 4,
+         float),
+        ('lap6_pbc_real', (12, 12, 12), (2.0, 2.0, 2.0), True, 'lap', 6,
+         float),
+        ('grad3x_pbc_real', (12, 10, 14), (3.0, 3.3, 3.6), True, 'g0', 3,
+         float),
+        ('grad3y_zero_real', (12, 10, 14), (3.0, 3.3, 3.6), False, 'g1', 3,
+         float),
+        ('grad2z_pbc_cplx', (8, 8, 12), (2.0, 2.0, 3.0), True, 'g2', 2,
+         complex),
+        ('guc_hex_lap2_real', (12, 12, 10), hexa, True, 'lap', 2, float),
+        ('guc_fcc_lap1_cplx', (8, 8, 8), fcc, True, 'lap', 1, complex),
+        ('mehrstellenA_real', (10, 12, 8), (2.0, 2.4, 1.6), True, 'A', 0,
+         float),
+    ]
+    for name, N, cell, pbc, kind, n, dtype in stencils:
+        gd = GridDescriptor(N, cell, pbc)
+        if kind == 'lap':
+            op = GUCLaplace(gd, scale=0.7, n=n, dtype=dtype)
+        elif kind == 'A':
+            op = LaplaceA(gd, -0.25 / np.pi, dtype=dtype)
+        else:
+            op = Gradient(gd, int(kind[1]), scale=1.3, n=n, dtype=dtype)
+        a = rand(rng, (2,) + tuple(gd.n_c), dtype)
+        ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+        b = np.zeros_like(a)
+        RefBackend().apply(spec(op), a, b, ph)
+        out[f'apply/{name}/coef'] = op.coef_p
+        out[f'apply/{name}/offset'] = op.offset_pc
+        out[f'apply/{name}/n_c'] = gd.n_c
+        out[f'apply/{name}/neighbor'] = gd.neighbor_cd
+        out[f'apply/{name}/phase'] = ph if ph is not None else np.zeros(0)
+        out[f'apply/{name}/in'] = a
+        out[f'apply/{name}/out'] = b
+
+    relaxes = [
+        ('lap3_pbc_w23', (12, 10, 14), True, 3, 3, 2 / 3),
+        ('lap1_zero_w1', (12, 10, 14), False, 1, 4, 1.0),
+        ('lap2_mixed_w07', (10, 12, 8), (False, True, True), 2, 2, 0.7),
+    ]
+    for name, N, pbc, n, sweeps, w in relaxes:
+        gd = GridDescriptor(N, (3.0, 3.3, 3.6), pbc)
+        op = Laplace(gd, -0.25 / np.pi, n)
+        f = rand(rng, tuple(gd.n_c), float)
+        s = rand(rng, tuple(gd.n_c), float)
+        f2 = f.copy()
+        RefBackend().relax(spec(op), f2, s, sweeps, w)
+        for key, val in [('coef', op.coef_p), ('offset', op.offset_pc),
+                         ('n_c', gd.n_c), ('neighbor', gd.neighbor_cd),
+                         ('f', f), ('src', s), ('out', f2),
+                         ('sweeps', np.array(sweeps)), ('w', np.array(w))]:
+            out[f'relax/{name}/{key}'] = val
+
+    transfers = []
+    for nn in (1, 2, 3, 4):
+        N = (16, 12, 20) if nn <= 2 else (24, 16, 20)
+        transfers.append((f'restrict_nn{nn}_pbc_real', N, True, nn, float,
+                          False))
+        transfers.append((f'interp_nn{nn}_pbc_real', N, True, nn, float,
+                          True))
+    transfers += [
+        ('restrict_nn1_zero_real', (16, 12, 20), False, 1, float, False),
+        ('interp_nn1_zero_real', (16, 12, 20), False, 1, float, True),
+        ('restrict_nn3_zero_real', (24, 16, 20), False, 3, float, False),
+        ('interp_nn3_zero_real', (24, 16, 20), False, 3, float, True),
+        ('restrict_nn2_bloch_cplx', (16, 12, 20), True, 2, complex, False),
+        ('interp_nn2_bloch_cplx', (16, 12, 20), True, 2, complex, True),
+        ('restrict_nn1_mixed_cplx', (16, 12, 20), (True, False, True), 1,
+         complex, False),
+        ('interp_nn4_mixed_cplx', (24, 16, 20), (True, True, False), 4,
+         complex, True),
+    ]
+    for name, N, pbc, nn, dtype, interp in transfers:
+        fine = GridDescriptor(N, (4.0, 3.0, 5.0), pbc)
+        coarse = fine.coarsen()
+        gin, gout = (coarse, fine) if interp else (fine, coarse)
+        t = Transformer(gin, gout, nn, dtype)
+        a = rand(rng, (1,) + tuple(gin.n_c), dtype)
+        ph = bloch_phases(rng, gin.pbc_c) if dtype == complex else None
+        b = np.zeros((1,) + tuple(gout.n_c), dtype)
+        RefBackend().transform(tspec(t), a, b, ph)
+        ts = tspec(t)
+        for key, val in [('interpolate', np.array(int(interp))),
+                         ('k', np.array(2 * nn)), ('size_in', ts.size_in),
+                         ('size_out', ts.size_out), ('pad', ts.pad_cd),
+                         ('npad', ts.npad_cd), ('skip', ts.skip_cd),
+                         ('neighbor', ts.neighbor_cd),
+                         ('phase', ph if ph is not None else np.zeros(0)),
+                         ('in', a), ('out', b)]:
+            out[f'transfer/{name}/{key}'] = val
+    return out
+
+
+def precond_cases():
+    from gpaw_amd.preconditioner import Preconditioner
+    rng = np.random.default_rng(7)
+    out = {}
+    cases = [('real_pbc', (16, 16, 16), True, float),
+             ('cplx_bloch', (16, 12, 16), True, complex),
+             ('real_zero', (16, 16, 16), False, float)]
+    for name, N, pbc, dtype in cases:
+        gd = GridDescriptor(N, (3.0, 2.4, 3.0), pbc)
+        kin0 = Laplace(gd, -0.5, 3, dtype)
+        P = Preconditioner(gd, kin0, dtype, block=3)
+        nb = 2
+        res = rand(rng, (nb,) + tuple(gd.n_c), dtype)
+        ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+        real = dtype == float
+        T = [ref_transformer(tspec(x), real) for x in
+             (P.restrictor_object0, P.restrictor_object1,
+              P.interpolator_object1, P.interpolator_object2)]
+        K = [ref_operator(spec(x), real) for x in (P.kin0, P.kin1, P.kin2)]
+        z = lambda g, m=1: np.zeros((m * nb,) + tuple(g.n_c), dtype)  # noqa
+        d0, q0 = z(gd), z(gd)
+        r1, d1, q1 = z(P.gd1), z(P.gd1), z(P.gd1)
+        r2, d2, q2 = z(P.gd2), z(P.gd2), z(P.gd2)
+        G.fd_precond(*T, *K, d0, q0, r1, d1, q1, r2, d2, q2, res, -res,
+                     P.step, ph if ph is not None else np.ones((3, 2), complex))
+        for key, val in [('N', np.array(N)), ('pbc', np.array(gd.pbc_c)),
+                         ('cell', np.array([3.0, 2.4, 3.0])),
+                         ('residuals', res), ('step', np.array(P.step)),
+                         ('phase', ph if ph is not None else np.zeros(0)),
+                         ('out', d0)]:
+            out[f'precond/{name}/{key}'] = val
+    return out
+
+
+def vcycle_of(solver):
+    solver._init()
+    ops = [spec(o) for o in solver.operators]
+    rst = [tspec(t) for t in solver.restrictors]
+    itp = [tspec(t) for t in solver.interpolators]
+    return ops, rst, itp
+
+
+def poisson_cases():
+    from gpaw_amd.poisson import FDPoissonSolver
+    out = {}
+    # 16^3 zero boundary, random neutral density, fixed 6 V-cycles
+    rng = np.random.default_rng(11)
+    gd = GridDescriptor((16, 16, 16), (6.0, 6.0, 6.0), False)
+    rho = rng.uniform(-1, 1, tuple(gd.n_c))
+    s = FDPoissonSolver(nn=3, eps=1e-30)
+    s.set_grid_descriptor(gd)
+    ops, rst, itp = vcycle_of(s)
+    V = oracle.PoissonVCycle(ops, rst, itp, s.presmooths, s.postsmooths,
+                             s.weights, gd.dv, backend=RefBackend())
+    phi = np.zeros(tuple(gd.n_c))
+    errs = []
+    V.phis[0] = phi
+    V.rhos[0][:] = rho
+    for _ in range(6):
+        errs.append(V.iterate2())
+    for key, val in [('N', np.array([16, 16, 16])), ('cell', np.array(6.0)),
+                     ('rho', rho), ('phi', phi), ('errors', np.array(errs))]:
+        out[f'poisson/zero16/{key}'] = val
+    # 48^3 periodic, 4 Gaussians, eps = 2e-10 (the reference's default)
+    gd, rho = workloads.poisson_48()
+    s = FDPoissonSolver(nn=3, eps=2e-10)
+    s.set_grid_descriptor(gd)
+    ops, rst, itp = vcycle_of(s)
+    V = oracle.PoissonVCycle(ops, rst, itp, s.presmooths, s.postsmooths,
+                             s.weights, gd.dv, backend=RefBackend())
+    background = gd.integrate(rho) / gd.dv / np.prod(gd.N_c)
+    phi = np.zeros(tuple(gd.n_c))
+    niter = V.solve_neutral(phi, rho - background, 2e-10, 1000, True)
+    for key, val in [('niter', np.array(niter)),
+                     ('error', np.array(V.last_error)),
+                     ('phi_sub', phi[::4, ::4, ::4].copy()),
+                     ('phi_sum2', np.array(float((phi**2).sum())))]:
+        out[f'poisson/pbc48/{key}'] = val
+    print('poisson_48: niter', niter, 'error', V.last_error)
+    return out
+
+
+def main():
+    np.savez_compressed(os.path.join(HERE, 'fd_golden.npz'), **fd_cases())
+    np.savez_compressed(os.path.join(HERE, 'precond_golden.npz'),
+                        **precond_cases())
+    np.savez_compressed(os.path.join(HERE, 'poisson_golden.npz'),
+                        **poisson_cases())
+    for f in sorted(glob.glob(os.path.join(HERE, '*.npz'))):
+        print(os.path.basename(f), os.path.getsize(f))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,52 @@
+"""Kinetic preconditioner (fd_precond chain) parity."""
+import numpy as np
+import pytest
+
+import oracle
+from gpaw_amd.fd_operators import Laplace
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.preconditioner import Preconditioner
+from gpu_util import dev, host, say
+from helpers import bloch_phases, case, rand, relerr, spec, tspec
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+class K:
+    def __init__(self, phase_cd):
+        self.phase_cd = phase_cd
+
+
+@pytest.mark.parametrize('name', ['real_pbc', 'cplx_bloch', 'real_zero'])
+def test_golden(golden, name):
+    c = case(golden('precond'), 'precond', name)
+    dtype = complex if c['residuals'].dtype == np.complex128 else float
+    gd = GridDescriptor(tuple(c['N']), tuple(c['cell']), tuple(c['pbc']))
+    P = Preconditioner(gd, Laplace(gd, -0.5, 3, dtype), dtype, block=2)
+    ph = None if c['phase'].size == 0 else c['phase']
+    say('golden precond', name)
+    out = host(P(dev(c['residuals']), K(ph)))
+    assert relerr(out, c['out']) < TOL
+
+
+def test_blocks_and_host_arrays():
+    rng = np.random.default_rng(5)
+    gd = GridDescriptor((24, 16, 32), (4.0, 3.0, 6.0))
+    dtype = complex
+    P = Preconditioner(gd, Laplace(gd, -0.5, 3, dtype), dtype, block=3)
+    res = rand(rng, (7,) + tuple(gd.n_c), dtype)
+    ph = bloch_phases(rng, gd.pbc_c)
+    ref = oracle.fd_precond(spec(P.kin0), spec(P.kin1), spec(P.kin2),
+                            tspec(P.restrictor_object0),
+                            tspec(P.restrictor_object1),
+                            tspec(P.interpolator_object1),
+                            tspec(P.interpolator_object2), res, P.step, ph)
+    say('precond blocks')
+    out = dev(np.zeros_like(res))
+    P(dev(res), K(ph), out=out)
+    assert relerr(host(out), ref) < TOL
+    out_h = P(res, K(ph), out=np.zeros_like(res))
+    assert relerr(out_h, ref) < TOL
+    one = host(P(dev(res[2]), K(ph)))
+    assert relerr(one, ref[2]) < TOL

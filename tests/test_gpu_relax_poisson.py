@@ -1,0 +1,119 @@
+"""Jacobi relaxation and the native multigrid Poisson driver against the
+reference fixtures and the oracle V-cycle."""
+import numpy as np
+import pytest
+
+import oracle
+from gpaw_amd import workloads
+from gpaw_amd.fd_operators import Laplace
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.poisson import FDPoissonSolver, PoissonConvergenceError
+from gpu_util import dev, general_path, host, say
+from helpers import case, case_names, rand, relerr, vcycle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+@pytest.mark.parametrize('path', ['fast', 'general'])
+def test_relax_golden(golden, path):
+    npz = golden('fd')
+    with general_path(path == 'general'):
+        for name in case_names(npz, 'relax'):
+            c = case(npz, 'relax', name)
+            pbc = tuple(bool(x >= 0) for x in c['neighbor'][:, 0])
+            N = tuple(int(n) + (0 if p else 1) for n, p in zip(c['n_c'], pbc))
+            gd = GridDescriptor(N, (1.0, 1.0, 1.0), pbc)
+            from gpaw_amd.fd_operators import FDOperator
+            op = FDOperator(c['coef'], c['offset'], gd)
+            f = dev(c['f'])
+            say('golden relax', name, path)
+            op.relax(2, f, dev(c['src']), int(c['sweeps']), float(c['w']))
+            assert relerr(host(f), c['out']) < TOL, name
+
+
+def test_relax_sweep_counts():
+    rng = np.random.default_rng(3)
+    gd = GridDescriptor((20, 24, 28), (4.0, 4.4, 5.0), (True, False, True))
+    op = Laplace(gd, -0.25 / np.pi, 3)
+    f0 = rand(rng, tuple(gd.n_c))
+    s = rand(rng, tuple(gd.n_c))
+    for n in (0, 1, 2, 5):
+        ref = f0.copy()
+        oracle.fd_relax(2, op.coef_p, op.offset_pc, gd.n_c, gd.neighbor_cd,
+                        ref, s, n, 2 / 3)
+        f = dev(f0)
+        say('relax sweeps', n)
+        op.relax(2, f, dev(s), n, 2 / 3)
+        assert relerr(host(f), ref) < TOL
+
+
+def test_gauss_seidel_not_on_gpu():
+    gd = GridDescriptor((8, 8, 8))
+    op = Laplace(gd, 1.0, 1)
+    f = dev(np.zeros((8, 8, 8)))
+    with pytest.raises(NotImplementedError, match='Gauss-Seidel'):
+        op.relax(1, f, f.clone(), 1)
+
+
+def test_vcycle_golden_zero16(golden):
+    c = case(golden('poisson'), 'poisson', 'zero16')
+    gd = GridDescriptor(tuple(c['N']), (float(c['cell']),) * 3, False)
+    s = FDPoissonSolver(nn=3, eps=1e-30)
+    s.set_grid_descriptor(gd)
+    phi = dev(np.zeros(tuple(gd.n_c)))
+    rho = dev(c['rho'])
+    errs = []
+    for i in range(6):
+        say('zero16 V-cycle', i)
+        errs.append(s.iterate2(s.step if hasattr(s, 'step') else 1.0,
+                               phi=phi, rho=rho))
+    assert relerr(host(phi), c['phi']) < TOL
+    assert errs == pytest.approx(list(c['errors']), rel=1e-11)
+
+
+def test_vcycle_matches_oracle_mixed_bc():
+    rng = np.random.default_rng(21)
+    gd = GridDescriptor((32, 16, 24), (6.0, 3.0, 4.5), (True, False, True))
+    rho = rand(rng, tuple(gd.n_c))
+    s = FDPoissonSolver(nn=2, eps=1e-30)
+    s.set_grid_descriptor(gd)
+    V = vcycle(s)
+    phi_ref = np.zeros(tuple(gd.n_c))
+    V.phis[0] = phi_ref
+    V.rhos[0][:] = rho
+    phi = dev(np.zeros(tuple(gd.n_c)))
+    drho = dev(rho)
+    for i in range(4):
+        say('mixed-bc V-cycle', i)
+        e_ref = V.iterate2()
+        e = s.iterate2(1.0, phi=phi, rho=drho)
+        assert e == pytest.approx(e_ref, rel=1e-10)
+        assert relerr(host(phi), phi_ref) < 1e-12
+
+
+def test_poisson_48_solve(golden):
+    c = case(golden('poisson'), 'poisson', 'pbc48')
+    gd, rho = workloads.poisson_48()
+    s = FDPoissonSolver(nn=3, eps=2e-10)
+    s.set_grid_descriptor(gd)
+    phi = dev(np.zeros(tuple(gd.n_c)))
+    say('poisson_48 solve')
+    niter = s.solve(phi, dev(rho))
+    assert niter == int(c['niter'])
+    assert s.last_error == pytest.approx(float(c['error']), rel=1e-9)
+    assert relerr(host(phi)[::4, ::4, ::4], c['phi_sub']) < TOL
+    # warm start from the converged potential
+    assert s.solve(phi, dev(rho)) == 1
+    # host arrays work too (staged through the device)
+    phi_h = np.zeros(tuple(gd.n_c))
+    assert s.solve(phi_h, rho) == int(c['niter'])
+    assert relerr(phi_h[::4, ::4, ::4], c['phi_sub']) < TOL
+
+
+def test_poisson_convergence_error():
+    gd, rho = workloads.poisson_48()
+    s = FDPoissonSolver(nn=3, eps=1e-30, maxiter=3)
+    s.set_grid_descriptor(gd)
+    with pytest.raises(PoissonConvergenceError):
+        s.solve(dev(np.zeros(tuple(gd.n_c))), dev(rho))

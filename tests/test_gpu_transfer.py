@@ -1,0 +1,89 @@
+"""HIP restriction / interpolation parity (Transformer.apply)."""
+import numpy as np
+import pytest
+
+import oracle
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.transformers import Transformer
+from gpu_util import dev, host, say
+from helpers import bloch_phases, case, case_names, rand, relerr, tspec
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def gpu_transfer(t, a, phases=None):
+    d = dev(a)
+    out = t.apply(d, None, phases)
+    return host(out)
+
+
+def ref_transfer(t, a, phases=None):
+    s = tspec(t)
+    return oracle.transform(s.interpolate, s.k, s.size_in, s.size_out,
+                            s.pad_cd, s.npad_cd, s.skip_cd, s.neighbor_cd, a,
+                            phases)
+
+
+def test_golden(golden):
+    npz = golden('fd')
+    for name in case_names(npz, 'transfer'):
+        c = case(npz, 'transfer', name)
+        say('golden transfer', name)
+        interp = bool(c['interpolate'])
+        nn = int(c['k']) // 2
+        pbc = tuple(bool(x >= 0) for x in c['neighbor'][:, 0])
+        fine_n = c['size_out'] if interp else c['size_in']
+        N = tuple(int(n) + (0 if p else 1) for n, p in zip(fine_n, pbc))
+        fine = GridDescriptor(N, (1.0, 1.0, 1.0), pbc)
+        gin, gout = (fine.coarsen(), fine) if interp else (fine,
+                                                           fine.coarsen())
+        dtype = complex if c['in'].dtype == np.complex128 else float
+        t = Transformer(gin, gout, nn, dtype)
+        ph = None if c['phase'].size == 0 else c['phase']
+        assert relerr(gpu_transfer(t, c['in'], ph), c['out']) < TOL, name
+
+
+@pytest.mark.parametrize('nn', [1, 2, 3, 4])
+def test_random(nn):
+    rng = np.random.default_rng(100 + nn)
+    for N in [(16, 12, 20), (24, 32, 40), (48, 16, 24)]:
+        for pbc in [True, False, (True, False, True), (False, False, True)]:
+            fine = GridDescriptor(N, np.array(N) * 0.2, pbc)
+            coarse = fine.coarsen()
+            for interp in (False, True):
+                gin, gout = (coarse, fine) if interp else (fine, coarse)
+                for dtype in (float, complex):
+                    try:
+                        t = Transformer(gin, gout, nn, dtype)
+                    except ValueError:
+                        continue
+                    a = rand(rng, (2,) + tuple(gin.n_c), dtype)
+                    ph = (bloch_phases(rng, gin.pbc_c)
+                          if dtype == complex else None)
+                    say('transfer', nn, N, pbc, interp, dtype.__name__)
+                    assert relerr(gpu_transfer(t, a, ph),
+                                  ref_transfer(t, a, ph)) < TOL
+
+
+@pytest.mark.parametrize('pbc', [True, False])
+@pytest.mark.parametrize('nn', [1, 2, 3, 4])
+def test_reference_gpu_case(pbc, nn):
+    """gpaw/test/gpu/test_fd_transformers.py: Gaussian on 32^3, restrict
+    then interpolate, GPU against CPU to 1e-14 absolute."""
+    lat = 8.0
+    gd = GridDescriptor((32, 32, 32), (lat, lat, lat), pbc_c=pbc)
+    dtype = complex if pbc else float
+    phase = np.ones((3, 2), complex) if pbc else None
+    x, y, z = gd.get_grid_point_coordinates()
+    mu = lat / 2.0
+    a = gd.zeros(dtype=dtype)
+    a[:] = np.exp(-((x - mu)**2 + (y - mu)**2 + (z - mu)**2) / 3.0)
+    coarse = gd.coarsen()
+    t = Transformer(gd, coarse, nn, dtype)
+    ac_ref = ref_transfer(t, a, phase)
+    say('reference gpu case', pbc, nn)
+    assert np.abs(gpu_transfer(t, a, phase) - ac_ref).max() < 1e-14
+    t = Transformer(coarse, gd, nn, dtype)
+    assert np.abs(gpu_transfer(t, ac_ref, phase) -
+                  ref_transfer(t, ac_ref, phase)).max() < 1e-14

@@ -1,0 +1,118 @@
+"""The oracle against the reference extension built from the reference
+sources (oracle/_ref), on randomised stencils, boundaries and transfers.
+Skipped where the reference build is absent (it never leaves the build
+container's tree of sources)."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.transformers import transformer_paddings
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), '..'))
+_SO = glob.glob(os.path.join(ROOT, 'oracle', '_ref', '_gpaw_ref*.so'))
+pytestmark = pytest.mark.skipif(not _SO, reason='oracle/_ref not built')
+
+
+@pytest.fixture(scope='module')
+def G():
+    sys.path.insert(0, os.path.dirname(_SO[0]))
+    import _gpaw_ref
+    return _gpaw_ref
+
+
+def ref_op(G, coef, off3, n, nb, real):
+    r = int(np.abs(off3).max())
+    M = np.array(n) + 2 * r
+    st = np.array([M[1] * M[2], M[2], 1])
+    return G.Operator(np.asarray(coef, float), np.ascontiguousarray(off3 @ st),
+                      np.array(n), r, np.array(nb).reshape(3, 2), int(real),
+                      None, int(oracle.is_cfd(off3)))
+
+
+def test_random_stencils_and_relax(G):
+    rng = np.random.default_rng(1)
+    for trial in range(30):
+        n = rng.integers(4, 11, 3)
+        r = int(rng.integers(1, 4))
+        P = int(rng.integers(2, 12))
+        off3 = rng.integers(-r, r + 1, (P, 3))
+        off3[0] = 0
+        if trial % 2:
+            for p in range(P):
+                ax = rng.integers(0, 3)
+                v = off3[p, ax]
+                off3[p] = 0
+                off3[p, ax] = v
+        if np.abs(off3).max() == 0:
+            off3[1, 0] = 1
+        coef = rng.standard_normal(P)
+        per = [0 if rng.random() < .6 else -1 for _ in range(3)]
+        nb = [[p, p] for p in per]
+        for real in (1, 0):
+            shp = (3,) + tuple(n)
+            a = rng.standard_normal(shp)
+            if not real:
+                a = a + 1j * rng.standard_normal(shp)
+            ph = np.exp(2j * np.pi * rng.random((3, 1)) * np.array([1, -1]))
+            b1 = np.empty_like(a)
+            op = ref_op(G, coef, off3, n, nb, real)
+            if real:
+                op.apply(a, b1)
+            else:
+                op.apply(a, b1, ph)
+            b2 = oracle.fd_apply(coef, off3, n, nb, a, None if real else ph)
+            assert (b1 == b2).all(), trial
+        coef[0] = -abs(coef).sum() * 2
+        op = ref_op(G, coef, off3, n, nb, 1)
+        for method in (1, 2):
+            f1 = rng.standard_normal(tuple(n))
+            s = rng.standard_normal(tuple(n))
+            f2 = f1.copy()
+            op.relax(method, f1, s, 3, 0.7)
+            oracle.fd_relax(method, coef, off3, n, nb, f2, s, 3, 0.7)
+            assert (f1 == f2).all(), (trial, method)
+
+
+def test_random_transfers(G):
+    rng = np.random.default_rng(2)
+    count = 0
+    for trial in range(60):
+        pbc = [bool(rng.random() < 0.6) for _ in range(3)]
+        N = rng.integers(3, 9, 3) * 2
+        gd = GridDescriptor(N, rng.random(3) * 5 + 3, pbc)
+        nn = int(rng.integers(1, 5))
+        for interp in (False, True):
+            gin, gout = (gd.coarsen(), gd) if interp else (gd, gd.coarsen())
+            try:
+                I, pad, npad, skip = transformer_paddings(gin, gout, nn)
+            except ValueError:
+                continue
+            for real in (0, 1):
+                shp = (1,) + tuple(gin.n_c)
+                a = rng.standard_normal(shp)
+                if not real:
+                    a = a + 1j * rng.standard_normal(shp)
+                ph = np.exp(2j * np.pi * rng.random((3, 1)) *
+                            np.array([1, -1]))
+                b2 = oracle.transform(I, 2 * nn, gin.n_c, gout.n_c, pad,
+                                      npad, skip, gin.neighbor_cd, a,
+                                      None if real else ph)
+                T = G.Transformer(gin.n_c, gout.n_c, 2 * nn, pad, npad, skip,
+                                  gin.neighbor_cd, real, None, int(I))
+                M = gin.n_c + pad.sum(1)
+                nout = int(np.prod(gout.n_c))
+                # room for the reference's first pass (see gen_golden.py)
+                big = np.zeros(nout + int(np.prod(M)) * 16, a.dtype)
+                b1 = big[:nout].reshape((1,) + tuple(gout.n_c))
+                if real:
+                    T.apply(a, b1)
+                else:
+                    T.apply(a, b1, ph)
+                assert (b1 == b2).all(), (trial, interp, real, nn)
+                count += 1
+    assert count > 100
