@@ -286,7 +286,7 @@ static int op_pass(gfd_operator* op, int dtype, const void* in, void* out,
                    hipStream_t s, int* nblocks)
 {
     const BC bc = make_bc(op->periodic, phases);
-    if (op->use_fast())
+    if (op->use_fast() && axis_stencil_fits(op->ax, dtype, op->n, nin))
         return launch_axis_stencil(op->ax, dtype, op->n, bc, in, out, nin, epi,
                                    ea, s, nblocks);
     return launch_general_stencil(op->gen(ea.c0), dtype, op->n, bc, in, out,
@@ -295,7 +295,7 @@ static int op_pass(gfd_operator* op, int dtype, const void* in, void* out,
 
 static long op_max_blocks(gfd_operator* op, int dtype, long nin)
 {
-    if (op->use_fast())
+    if (op->use_fast() && axis_stencil_fits(op->ax, dtype, op->n, nin))
         return axis_stencil_max_blocks(op->ax, dtype, op->n, nin);
     return general_stencil_max_blocks(op->n, nin);
 }

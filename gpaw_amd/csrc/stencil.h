@@ -56,6 +56,10 @@ int launch_general_stencil(const GeneralStencil& st, int dtype,
                            void* out, long nin, int epi, const EpiArgs& ea,
                            hipStream_t stream, int* nblocks_out);
 
+// Whether the x-marching kernel takes this stencil / grid (else general).
+bool axis_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
+                       long nin);
+
 // Block-count upper bounds for the residual-norm partial buffer.
 long axis_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
                              long nin);

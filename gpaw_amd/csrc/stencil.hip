@@ -485,6 +485,16 @@ void xm_set_override(const XmTile* t)
 // rows per thread: complex R >= 3 with 4 rows exceeds 256 VGPRs
 static int py_for(int R, int dtype) { return (dtype == 1 && R >= 2) ? 2 : 4; }
 
+static bool xm_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
+{
+    const int NT = t.tz * t.nseg;
+    const int TY = t.nseg * t.py;
+    const int es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);
+    return NT <= XM_MAXT && 2 * R * (t.tz + TY) <= XM_HMAX * NT &&
+           t.tz <= n[2] && R <= n[0] && R <= n[1] && R <= n[2] &&
+           2L * (TY + 2 * R) * (t.tz + 2 * R) * es <= 150 * 1024;
+}
+
 XmTile xm_default_tile(int R, int dtype, const int n[3], long nin)
 {
     {
@@ -494,18 +504,19 @@ XmTile xm_default_tile(int R, int dtype, const int n[3], long nin)
     XmTile t;
     const int ntz = (n[2] + 127) / 128;
     t.tz = (n[2] + ntz - 1) / ntz;
-    t.py = py_for(R, dtype);
-    if (t.py > n[1]) t.py = 1;
-    // rows: as many segments as fit in XM_MAXT threads, but no more rows
-    // than the grid has
-    int maxseg = std::max(1, XM_MAXT / t.tz);
-    int want = (n[1] + t.py - 1) / t.py;
-    t.nseg = std::max(1, std::min(maxseg, want));
-    // keep the LDS footprint modest for complex data
     const int es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);
-    while (t.nseg > 1 &&
-           2L * (t.nseg * t.py + 2 * R) * (t.tz + 2 * R) * es > 96 * 1024)
-        t.nseg--;
+    // rows per thread: the largest that fits the halo budget and registers
+    for (int py = py_for(R, dtype); py >= 1; py >>= 1) {
+        t.py = py;
+        const int maxseg = std::max(1, XM_MAXT / t.tz);
+        const int want = (n[1] + t.py - 1) / t.py;
+        t.nseg = std::max(1, std::min(maxseg, want));
+        while (t.nseg > 1 &&
+               2L * (t.nseg * t.py + 2 * R) * (t.tz + 2 * R) * es > 96 * 1024)
+            t.nseg--;
+        t.xchunk = n[0];
+        if (xm_tile_ok(t, R, dtype, n)) break;
+    }
     const long tiles = (long)ceil_div(n[1], t.nseg * t.py) *
                        ceil_div(n[2], t.tz) * nin;
     int chunks = 1;
@@ -516,6 +527,12 @@ XmTile xm_default_tile(int R, int dtype, const int n[3], long nin)
     }
     t.xchunk = ceil_div(n[0], chunks);
     return t;
+}
+
+bool axis_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
+                       long nin)
+{
+    return xm_tile_ok(xm_default_tile(st.R, dtype, n, nin), st.R, dtype, n);
 }
 
 template <typename T, int R, int PY, int EPI>
@@ -569,13 +586,21 @@ static int launch_xm_t(const AxisStencil& st, const XmTile& tl,
     }
     const size_t shm =
         a.has_lds ? 2 * (size_t)(TY + 2 * R) * (tl.tz + 2 * R) * sizeof(T) : 0;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void*)k_xmarch<T, R, PY, EPI>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
-        attr_set = true;
+    // more than the default 64 KB of dynamic LDS must be opted into
+    static size_t attr_bytes = 64 * 1024;
+    if (shm > attr_bytes) {
+        hipError_t e = hipFuncSetAttribute(
+            (const void*)k_xmarch<T, R, PY, EPI>,
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("x-march: %zu bytes of LDS not available (%s)", shm,
+                      hipGetErrorString(e));
+            return ENOTSUP_;
+        }
+        attr_bytes = shm;
     }
+    (void)hipGetLastError();
     hipLaunchKernelGGL((k_xmarch<T, R, PY, EPI>), dim3((unsigned)nb),
                        dim3(tl.tz, tl.nseg), shm, stream, a);
     GFD_LAUNCH_CHECK();

@@ -128,7 +128,7 @@ def test_argument_errors():
     with pytest.raises(TypeError):
         op.apply(a.to(torch.complex128), a.to(torch.complex128))
     with pytest.raises(ValueError):
-        op.apply(a.t(), a)
+        op.apply(a.transpose(0, 2), a)
 
 
 def test_residual_norm():
