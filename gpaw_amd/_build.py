@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, 'libgpaw_fd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = 'gfx950'
 
-SOURCES = ['api.hip', 'stencil.hip', 'transfer.hip', 'vector.hip']
+SOURCES = ['api.hip', 'stencil.hip', 'stencil_rows.hip', 'transfer.hip',
+           'vector.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',

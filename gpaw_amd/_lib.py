@@ -51,6 +51,7 @@ SIGNATURES = {
     'gfd_poisson_solve': (_i, [_vp, _vp, _vp, _d, _i, _i, _ip, _dp, _vp]),
     'gfd_dot': (_i, [_vp, _vp, _l, _dp, _vp]),
     'gfd_add_scalar': (_i, [_vp, _l, _d, _vp]),
+    'gfd_stream_copy': (_i, [_vp, _vp, _l, _vp]),
     'gfd_set_tuning': (_i, [ctypes.c_char_p, _l]),
 }
 

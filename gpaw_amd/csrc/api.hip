@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstring>
@@ -31,7 +32,8 @@ void set_error(const char* fmt, ...)
 
 const char* last_error() { return g_err.c_str(); }
 
-static bool g_force_general = false;
+// 0 auto, 1 general kernel, 2 tiled x-march kernel, 3 full-row kernel
+static int g_force_path = 0;
 
 // Grow-only device buffer.  Growing waits for the stream first, so a buffer
 // still read by queued kernels is never freed under them.
@@ -127,7 +129,7 @@ struct gfd_operator {
         g.off = d_off;
         return g;
     }
-    bool use_fast() const { return axis_ok && !g_force_general; }
+    bool use_fast() const { return axis_ok && g_force_path != 1; }
 };
 
 extern "C" const char* gfd_last_error(void) { return gfd::last_error(); }
@@ -280,24 +282,53 @@ extern "C" double gfd_operator_get_diagonal_element(const gfd_operator* op)
     return op ? op->diag : 0.0;
 }
 
+static bool aligned16(const void* p)
+{
+    return ((uintptr_t)p & 15) == 0;
+}
+
+// Kernel family for one pass: 3 full-row, 2 tiled x-march, 1 general.
+static int op_path(gfd_operator* op, int dtype, long nin, const void* in,
+                   const void* out, const void* aux)
+{
+    if (!op->use_fast()) return 1;
+    const bool row = (g_force_path == 0 || g_force_path == 3) &&
+                     aligned16(in) && aligned16(out) &&
+                     (aux == nullptr || aligned16(aux)) &&
+                     row_stencil_fits(op->ax, dtype, op->n, nin);
+    if (row) return 3;
+    if ((g_force_path == 0 || g_force_path == 2) &&
+        axis_stencil_fits(op->ax, dtype, op->n, nin))
+        return 2;
+    return 1;
+}
+
 // One stencil pass with an epilogue (the building block of every driver).
 static int op_pass(gfd_operator* op, int dtype, const void* in, void* out,
                    long nin, const double* phases, int epi, EpiArgs ea,
                    hipStream_t s, int* nblocks)
 {
     const BC bc = make_bc(op->periodic, phases);
-    if (op->use_fast() && axis_stencil_fits(op->ax, dtype, op->n, nin))
+    const int path = op_path(op, dtype, nin, in, out, ea.aux);
+    if (path == 3)
+        return launch_row_stencil(op->ax, dtype, op->n, bc, in, out, nin, epi,
+                                  ea, s, nblocks);
+    if (path == 2)
         return launch_axis_stencil(op->ax, dtype, op->n, bc, in, out, nin, epi,
                                    ea, s, nblocks);
     return launch_general_stencil(op->gen(ea.c0), dtype, op->n, bc, in, out,
                                   nin, epi, ea, s, nblocks);
 }
 
+// Upper bound of the blocks a residual pass may launch (partial sums).
 static long op_max_blocks(gfd_operator* op, int dtype, long nin)
 {
-    if (op->use_fast() && axis_stencil_fits(op->ax, dtype, op->n, nin))
-        return axis_stencil_max_blocks(op->ax, dtype, op->n, nin);
-    return general_stencil_max_blocks(op->n, nin);
+    long m = general_stencil_max_blocks(op->n, nin);
+    if (op->use_fast()) {
+        m = std::max(m, axis_stencil_max_blocks(op->ax, dtype, op->n, nin));
+        m = std::max(m, row_stencil_max_blocks(op->ax, dtype, op->n, nin));
+    }
+    return m;
 }
 
 static double centre_coef(const gfd_operator* op)
@@ -899,6 +930,17 @@ extern "C" int gfd_add_scalar(double* a, long n, double v, void* stream)
     return vec_add_scalar(a, n, v, (hipStream_t)stream);
 }
 
+extern "C" int gfd_stream_copy(const void* src, void* dst, long nbytes,
+                               void* stream)
+{
+    if (!src || !dst || nbytes < 0 || (nbytes & 15) ||
+        ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) {
+        set_error("gfd_stream_copy: 16-byte aligned pointers and sizes");
+        return EINVAL_;
+    }
+    return vec_copy16(src, dst, nbytes, (hipStream_t)stream);
+}
+
 extern "C" int gfd_set_tuning(const char* key, long value)
 {
     if (!key) {
@@ -906,7 +948,32 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         return EINVAL_;
     }
     if (std::strcmp(key, "force_general") == 0) {
-        g_force_general = value != 0;
+        g_force_path = value ? 1 : 0;
+        return OK;
+    }
+    if (std::strcmp(key, "force_path") == 0) {
+        if (value < 0 || value > 3) {
+            set_error("gfd_set_tuning: force_path must be 0..3");
+            return EINVAL_;
+        }
+        g_force_path = (int)value;
+        return OK;
+    }
+    if (std::strcmp(key, "xr_tile") == 0) {
+        if (value == 0) {
+            xr_set_override(nullptr);
+            return OK;
+        }
+        XmTile t;
+        t.tz = (int)(value & 0xfff);
+        t.nseg = (int)((value >> 12) & 0xfff);
+        t.py = (int)((value >> 24) & 0xff);
+        t.xchunk = (int)((value >> 32) & 0xffff);
+        if (t.tz <= 0 || t.nseg <= 0 || t.py <= 0 || t.xchunk <= 0) {
+            set_error("gfd_set_tuning: bad xr_tile");
+            return EINVAL_;
+        }
+        xr_set_override(&t);
         return OK;
     }
     if (std::strcmp(key, "xm_tile") == 0) {

@@ -45,6 +45,13 @@ struct GeneralStencil {
     const int* off;           // [P][3]
 };
 
+// Tile choice of the x-marching kernel (exposed for the autotuner).
+struct XmTile {
+    int tz;      // tile width along z (threads per row)
+    int nseg;    // row segments (threads along y)
+    int py;      // rows per thread
+    int xchunk;  // planes per block along x
+};
 // Launch one stencil pass over `nin` grids of shape n[3].
 // dtype: 0 = f64, 1 = c128, 2 = f32.
 int launch_axis_stencil(const AxisStencil& st, int dtype, const int n[3],
@@ -55,6 +62,19 @@ int launch_general_stencil(const GeneralStencil& st, int dtype,
                            const int n[3], const BC& bc, const void* in,
                            void* out, long nin, int epi, const EpiArgs& ea,
                            hipStream_t stream, int* nblocks_out);
+
+// Full-row x-marching kernel (16-byte vector accesses; n2 % (16 / element
+// size) == 0): the first choice for axis-only stencils.
+int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
+                       const BC& bc, const void* in, void* out, long nin,
+                       int epi, const EpiArgs& ea, hipStream_t stream,
+                       int* nblocks_out);
+bool row_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
+                      long nin);
+long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
+                            long nin);
+XmTile xr_default_tile(int R, int dtype, const int n[3], long nin);
+void xr_set_override(const XmTile* t);
 
 // Whether the x-marching kernel takes this stencil / grid (else general).
 bool axis_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
@@ -69,13 +89,6 @@ long general_stencil_max_blocks(const int n[3], long nin);
 int launch_sum_partials(const double* partial, long np, double* result,
                         hipStream_t stream);
 
-// Tile choice of the x-marching kernel (exposed for the autotuner).
-struct XmTile {
-    int tz;      // tile width along z (threads per row)
-    int nseg;    // row segments (threads along y)
-    int py;      // rows per thread
-    int xchunk;  // planes per block along x
-};
 XmTile xm_default_tile(int R, int dtype, const int n[3], long nin);
 void xm_set_override(const XmTile* t);  // nullptr clears
 

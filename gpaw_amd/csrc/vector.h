@@ -8,6 +8,7 @@ namespace gfd {
 int vec_zero(double* a, long n, hipStream_t s);
 int vec_copy(const double* a, double* b, long n, hipStream_t s);
 int vec_add_scalar(double* a, long n, double v, hipStream_t s);
+int vec_copy16(const void* a, void* b, long nbytes, hipStream_t s);
 long vec_dot_partials(long n);
 // *dev_result = sum_i a_i b_i (b == nullptr: sum_i a_i), deterministic order
 int vec_dot(const double* a, const double* b, long n, double* partial,

@@ -50,6 +50,15 @@ k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// 16-byte streaming copy (the achievable-bandwidth reference)
+__global__ void __launch_bounds__(256)
+k_copy16(const double2* __restrict__ a, double2* __restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        b[i] = a[i];
+}
+
 static unsigned grid_for(long n)
 {
     long b = (n + 255) / 256;
@@ -79,6 +88,18 @@ int vec_add_scalar(double* a, long n, double v, hipStream_t s)
     if (n <= 0) return OK;
     hipLaunchKernelGGL(k_add_scalar, dim3(grid_for(n)), dim3(256), 0, s, a,
                        n, v);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_copy16(const void* a, void* b, long nbytes, hipStream_t s)
+{
+    const long n = nbytes / 16;
+    if (n <= 0) return OK;
+    long g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(256), 0, s,
+                       (const double2*)a, (double2*)b, n);
     GFD_LAUNCH_CHECK();
     return OK;
 }

@@ -145,10 +145,18 @@ int gfd_dot(const double* a, const double* b, long n, double* result,
 /* a_i += v for n doubles (the `phi -= phi_ave` of gpaw/poisson.py:496). */
 int gfd_add_scalar(double* a, long n, double v, void* stream);
 
-/* Tuning knobs (no reference counterpart): key "force_general" (0/1) routes
- * every stencil through the general kernel; "xm_tile" with value
- * tz | nseg << 12 | py << 24 | xchunk << 32 overrides the x-march tile
- * (value 0 clears).  Returns GFD_EINVAL for unknown keys. */
+/* Device-to-device copy of nbytes with 16-byte accesses (no reference
+ * counterpart): the measured streaming bandwidth that stencil throughput is
+ * compared against.  16-byte aligned pointers, nbytes a multiple of 16. */
+int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
+
+/* Tuning knobs (no reference counterpart): key "force_path" selects the
+ * stencil kernel family (0 automatic, 1 general, 2 tiled x-march, 3
+ * full-row x-march; a family that cannot take a grid falls back to the
+ * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
+ * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
+ * the tile of the tiled / full-row x-march kernel (value 0 clears).
+ * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 
 #ifdef __cplusplus

@@ -20,12 +20,20 @@ def say(*args):
     print(*args, flush=True, file=sys.stderr)
 
 
+PATHS = {'auto': 0, 'general': 1, 'xmarch': 2, 'row': 3}
+
+
 @contextlib.contextmanager
-def general_path(on=True):
-    """Route every stencil through the general kernel while active."""
+def kernel_path(name='general'):
+    """Route every stencil through one kernel family while active (a
+    family that cannot take a grid falls back to the general kernel)."""
     lib = _lib.load()
-    lib.gfd_set_tuning(b'force_general', int(on))
+    lib.gfd_set_tuning(b'force_path', PATHS[name])
     try:
         yield
     finally:
-        lib.gfd_set_tuning(b'force_general', 0)
+        lib.gfd_set_tuning(b'force_path', 0)
+
+
+def general_path(on=True):
+    return kernel_path('general' if on else 'auto')

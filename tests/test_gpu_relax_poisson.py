@@ -8,17 +8,17 @@ from gpaw_amd import workloads
 from gpaw_amd.fd_operators import Laplace
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpaw_amd.poisson import FDPoissonSolver, PoissonConvergenceError
-from gpu_util import dev, general_path, host, say
+from gpu_util import dev, host, kernel_path, say
 from helpers import case, case_names, rand, relerr, vcycle
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
 
-@pytest.mark.parametrize('path', ['fast', 'general'])
+@pytest.mark.parametrize('path', ['row', 'xmarch', 'general'])
 def test_relax_golden(golden, path):
     npz = golden('fd')
-    with general_path(path == 'general'):
+    with kernel_path(path):
         for name in case_names(npz, 'relax'):
             c = case(npz, 'relax', name)
             pbc = tuple(bool(x >= 0) for x in c['neighbor'][:, 0])
@@ -42,10 +42,12 @@ def test_relax_sweep_counts():
         ref = f0.copy()
         oracle.fd_relax(2, op.coef_p, op.offset_pc, gd.n_c, gd.neighbor_cd,
                         ref, s, n, 2 / 3)
-        f = dev(f0)
-        say('relax sweeps', n)
-        op.relax(2, f, dev(s), n, 2 / 3)
-        assert relerr(host(f), ref) < TOL
+        for path in ('row', 'xmarch', 'general'):
+            f = dev(f0)
+            say('relax sweeps', n, path)
+            with kernel_path(path):
+                op.relax(2, f, dev(s), n, 2 / 3)
+            assert relerr(host(f), ref) < TOL
 
 
 def test_gauss_seidel_not_on_gpu():

@@ -6,7 +6,7 @@ import pytest
 import oracle
 from gpaw_amd.fd_operators import FDOperator, GUCLaplace, Gradient, Laplace
 from gpaw_amd.grid_descriptor import GridDescriptor
-from gpu_util import dev, general_path, host, say
+from gpu_util import dev, host, kernel_path, say
 from helpers import bloch_phases, case, case_names, rand, relerr
 
 pytestmark = pytest.mark.gpu
@@ -25,10 +25,10 @@ def ref_apply(op, a, phase=None):
                            op.gd.neighbor_cd, a, phase)
 
 
-@pytest.mark.parametrize('path', ['fast', 'general'])
+@pytest.mark.parametrize('path', ['row', 'xmarch', 'general'])
 def test_golden(golden, path):
     npz = golden('fd')
-    with general_path(path == 'general'):
+    with kernel_path(path):
         for name in case_names(npz, 'apply'):
             c = case(npz, 'apply', name)
             say('golden apply', name, path)
@@ -68,9 +68,9 @@ def test_laplace_random(nn, dtype):
             ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
             say('laplace', nn, dtype.__name__, N, pbc)
             ref = ref_apply(op, a, ph)
-            assert relerr(gpu_apply(op, a, ph), ref) < TOL
-            with general_path():
-                assert relerr(gpu_apply(op, a, ph), ref) < TOL
+            for path in ('row', 'xmarch', 'general'):
+                with kernel_path(path):
+                    assert relerr(gpu_apply(op, a, ph), ref) < TOL, path
 
 
 @pytest.mark.parametrize('v', [0, 1, 2])
@@ -83,7 +83,10 @@ def test_gradient(v):
             a = rand(rng, (3,) + tuple(gd.n_c), dtype)
             ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
             say('gradient', v, N, pbc, dtype.__name__)
-            assert relerr(gpu_apply(op, a, ph), ref_apply(op, a, ph)) < TOL
+            ref = ref_apply(op, a, ph)
+            for path in ('row', 'xmarch', 'general'):
+                with kernel_path(path):
+                    assert relerr(gpu_apply(op, a, ph), ref) < TOL, path
 
 
 def test_guc_and_general_offsets():
@@ -141,6 +144,8 @@ def test_residual_norm():
     d = dev(phi)
     r = d.new_empty(d.shape)
     say('residual norm')
-    n2 = op.residual(d, dev(rho), r, norm=True)
-    assert relerr(host(r), r_ref) < TOL
-    assert n2 == pytest.approx(float((r_ref**2).sum()), rel=1e-12)
+    for path in ('row', 'xmarch', 'general'):
+        with kernel_path(path):
+            n2 = op.residual(d, dev(rho), r, norm=True)
+            assert relerr(host(r), r_ref) < TOL
+            assert n2 == pytest.approx(float((r_ref**2).sum()), rel=1e-12)
