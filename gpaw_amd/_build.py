@@ -17,7 +17,7 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = 'gfx950'
 
 SOURCES = ['api.hip', 'stencil.hip', 'stencil_rows.hip', 'transfer.hip',
-           'vector.hip']
+           'vector.hip', 'bands.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',
@@ -47,6 +47,8 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', f'--offload-arch={ARCH}', '-o', tmp]
     cmd += objs
+    cmd += ['-L/opt/rocm/lib', '-lrocsolver', '-lrocblas',
+            '-Wl,-rpath,/opt/rocm/lib']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'link failed:\n{r.stderr[-4000:]}')

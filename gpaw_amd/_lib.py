@@ -49,6 +49,13 @@ SIGNATURES = {
     'gfd_poisson_free': (None, [_vp]),
     'gfd_poisson_iterate': (_i, [_vp, _vp, _vp, _dp, _vp]),
     'gfd_poisson_solve': (_i, [_vp, _vp, _vp, _d, _i, _i, _ip, _dp, _vp]),
+    'gfd_poisson_set_graph': (_i, [_vp, _i]),
+    'gfd_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _i, _d,
+                                     _d, _i, _i, _ip, _dp, _vp]),
+    'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
+    'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
+    'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),
+    'gfd_band_cholesky_orth': (_i, [_vp, _i, _vp, _l, _l, _vp, _vp]),
     'gfd_dot': (_i, [_vp, _vp, _l, _dp, _vp]),
     'gfd_add_scalar': (_i, [_vp, _l, _d, _vp]),
     'gfd_stream_copy': (_i, [_vp, _vp, _l, _vp]),

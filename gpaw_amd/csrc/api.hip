@@ -15,6 +15,7 @@
 #include "stencil.h"
 #include "transfer.h"
 #include "vector.h"
+#include "bands.h"
 
 namespace gfd {
 
@@ -289,7 +290,7 @@ static bool aligned16(const void* p)
 
 // Kernel family for one pass: 3 full-row, 2 tiled x-march, 1 general.
 static int op_path(gfd_operator* op, int dtype, long nin, const void* in,
-                   const void* out, const void* aux)
+                   const void* out, const void* aux, int epi)
 {
     if (!op->use_fast()) return 1;
     const bool row = (g_force_path == 0 || g_force_path == 3) &&
@@ -297,7 +298,7 @@ static int op_path(gfd_operator* op, int dtype, long nin, const void* in,
                      (aux == nullptr || aligned16(aux)) &&
                      row_stencil_fits(op->ax, dtype, op->n, nin);
     if (row) return 3;
-    if ((g_force_path == 0 || g_force_path == 2) &&
+    if (epi != EPI_POT && (g_force_path == 0 || g_force_path == 2) &&
         axis_stencil_fits(op->ax, dtype, op->n, nin))
         return 2;
     return 1;
@@ -309,7 +310,7 @@ static int op_pass(gfd_operator* op, int dtype, const void* in, void* out,
                    hipStream_t s, int* nblocks)
 {
     const BC bc = make_bc(op->periodic, phases);
-    const int path = op_path(op, dtype, nin, in, out, ea.aux);
+    const int path = op_path(op, dtype, nin, in, out, ea.aux, epi);
     if (path == 3)
         return launch_row_stencil(op->ax, dtype, op->n, bc, in, out, nin, epi,
                                   ea, s, nblocks);
@@ -580,14 +581,23 @@ extern "C" void gfd_transformer_free(gfd_transformer* t)
     delete t;
 }
 
+static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
+                   long nin, const double* phases, const XEpiArgs& ea,
+                   hipStream_t s)
+{
+    const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);
+    if (t->plan.K != 2)
+        GFD_TRY(t->scratch.ensure(es * xfer_scratch_elems(t->plan, nin) + 16,
+                                  s));
+    const BC bc = make_bc(t->periodic, t->real ? nullptr : phases);
+    return launch_transfer(t->plan, dtype, bc, in, out, nin, t->scratch.p, ea,
+                           s);
+}
+
 static int xfer(gfd_transformer* t, const void* in, void* out, long nin,
                 const double* phases, const XEpiArgs& ea, hipStream_t s)
 {
-    const size_t es = t->real ? 8 : 16;
-    GFD_TRY(t->scratch.ensure(es * xfer_scratch_elems(t->plan, nin) + 16, s));
-    const BC bc = make_bc(t->periodic, t->real ? nullptr : phases);
-    return launch_transfer(t->plan, t->dtype(), bc, in, out, nin, t->scratch.p,
-                           ea, s);
+    return xfer_dt(t, t->dtype(), in, out, nin, phases, ea, s);
 }
 
 extern "C" int gfd_transformer_apply(gfd_transformer* t, const void* in,
@@ -713,6 +723,14 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
 
 // ================================================================= Poisson
 
+// Level buffers of one precision (dtype 0: f64, 2: f32).
+struct Levels {
+    int dtype = 0;
+    int L = -1;  // coarsest level index held
+    DevBuf pool;
+    std::vector<void*> phi, tmp, rho, res;
+};
+
 struct gfd_poisson {
     int L = 0;  // number of coarsenings
     std::vector<gfd_operator*> ops;
@@ -720,10 +738,44 @@ struct gfd_poisson {
     std::vector<int> pre, post;
     std::vector<double> w;
     double dv = 1.0;
-    DevBuf pool;  // phi, tmp, rho, residual of every level
-    std::vector<double*> phi, tmp, rho, res;
-    DevBuf scalar;
+    Levels lv;     // f64 levels (phi[0], rho[0]: owned copies for graphs)
+    Levels lv32;   // f32 levels of the mixed-precision inner cycles
+    DevBuf scalar; // norms
+    // graph replay of one V-cycle (+ residual norm) on an owned stream
+    bool use_graph = true;
+    hipStream_t gs = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t g64 = nullptr;
+    hipGraphExec_t g32 = nullptr;
+    int g32_key[6] = {-1, -1, -1, -1, -1, -1};
+    double g32_w[2] = {0, 0};
 };
+
+static int alloc_levels(gfd_poisson* p, Levels& lv, int dtype, int L)
+{
+    if (lv.L >= L && lv.dtype == dtype) return OK;
+    const size_t es = dtype == 2 ? 4 : 8;
+    long total = 0;
+    for (int l = 0; l <= L; l++) total += 4 * ((p->ops[l]->ng() + 3) & ~3L);
+    lv.pool.release();
+    GFD_TRY(lv.pool.ensure(es * total, nullptr));
+    lv.phi.clear();
+    lv.tmp.clear();
+    lv.rho.clear();
+    lv.res.clear();
+    char* q = (char*)lv.pool.p;
+    for (int l = 0; l <= L; l++) {
+        const size_t nb = es * ((p->ops[l]->ng() + 3) & ~3L);
+        lv.phi.push_back(q);
+        lv.tmp.push_back(q + nb);
+        lv.rho.push_back(q + 2 * nb);
+        lv.res.push_back(q + 3 * nb);
+        q += 4 * nb;
+    }
+    lv.dtype = dtype;
+    lv.L = L;
+    return OK;
+}
 
 extern "C" int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
                                gfd_transformer* const* restrictors,
@@ -770,78 +822,194 @@ extern "C" int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
     p->post.assign(postsmooths, postsmooths + nlevels + 1);
     p->w.assign(weights, weights + nlevels + 1);
     p->dv = dv;
-    long total = 0;
-    for (int l = 0; l <= nlevels; l++) total += 4 * p->ops[l]->ng();
-    int st = p->pool.ensure(sizeof(double) * total, nullptr);
+    int st = alloc_levels(p, p->lv, 0, nlevels);
     if (st != OK) {
         delete p;
         return st;
-    }
-    double* q = (double*)p->pool.p;
-    for (int l = 0; l <= nlevels; l++) {
-        const long ng = p->ops[l]->ng();
-        p->phi.push_back(q);
-        p->tmp.push_back(q + ng);
-        p->rho.push_back(q + 2 * ng);
-        p->res.push_back(q + 3 * ng);
-        q += 4 * ng;
     }
     *out = p;
     return OK;
 }
 
+static void drop_graphs(gfd_poisson* p)
+{
+    if (p->g64) (void)hipGraphExecDestroy(p->g64);
+    if (p->g32) (void)hipGraphExecDestroy(p->g32);
+    p->g64 = nullptr;
+    p->g32 = nullptr;
+}
+
 extern "C" void gfd_poisson_free(gfd_poisson* p)
 {
     if (!p) return;
-    p->pool.release();
+    drop_graphs(p);
+    if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+    if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    if (p->gs) (void)hipStreamDestroy(p->gs);
+    p->lv.pool.release();
+    p->lv32.pool.release();
     p->scalar.release();
     delete p;
 }
 
-// gpaw/poisson.py:500-526 for one level; *cur/*alt as in relax_pp.
-static int vcycle(gfd_poisson* p, int l, double** cur, double** alt,
-                  const double* rho, hipStream_t s)
+// nrelax Jacobi sweeps of any precision ping-ponging between *cur / *alt
+static int relax_pp_t(gfd_operator* op, int dtype, void** cur, void** alt,
+                      const void* src, int nrelax, double w, hipStream_t s)
+{
+    EpiArgs ea{};
+    ea.aux = src;
+    ea.c0 = op->coef[0];
+    ea.w = w;
+    for (int k = 0; k < nrelax; k++) {
+        GFD_TRY(op_pass(op, dtype, *cur, *alt, 1, nullptr, EPI_RELAX, ea, s,
+                        nullptr));
+        std::swap(*cur, *alt);
+    }
+    return OK;
+}
+
+// Smoothing schedule of one V-cycle.
+struct Schedule {
+    int coarsest;   // level treated as the coarsest
+    const int* pre;
+    const int* post;
+    const double* w;
+    int pre_c, post_c;       // schedule override (>= 0) for l < coarsest
+    int coarse_sweeps;       // sweeps at the coarsest level (>= 0: override)
+    double w0, wc;           // weights override (> 0) for level 0 / others
+};
+
+static int sched_pre(const Schedule& sc, int l)
+{
+    return sc.pre_c >= 0 ? sc.pre_c : sc.pre[l];
+}
+static int sched_post(const Schedule& sc, int l)
+{
+    if (l == sc.coarsest && sc.coarse_sweeps >= 0) return sc.coarse_sweeps;
+    return sc.post_c >= 0 ? sc.post_c : sc.post[l];
+}
+static double sched_w(const Schedule& sc, int l)
+{
+    if (l == 0 && sc.w0 > 0) return sc.w0;
+    if (l > 0 && sc.wc > 0) return sc.wc;
+    return sc.w[l];
+}
+
+// gpaw/poisson.py:500-526 for one level; *cur / *alt hold phi_l.
+static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
+                  void** cur, void** alt, const void* rho, hipStream_t s)
 {
     gfd_operator* op = p->ops[l];
-    if (l < p->L) {
-        GFD_TRY(relax_pp(op, cur, alt, rho, p->pre[l], p->w[l], s));
+    const int dt = lv.dtype;
+    const size_t es = dt == 2 ? 4 : 8;
+    if (l < sc.coarsest) {
+        GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_pre(sc, l),
+                           sched_w(sc, l), s));
         EpiArgs e{};
         e.aux = rho;
         e.c0 = centre_coef(op);
-        GFD_TRY(op_pass(op, 0, *cur, p->res[l], 1, nullptr, EPI_RESID, e, s,
+        GFD_TRY(op_pass(op, dt, *cur, lv.res[l], 1, nullptr, EPI_RESID, e, s,
                         nullptr));
         XEpiArgs xe;
-        GFD_TRY(xfer(p->rst[l], p->res[l], p->rho[l + 1], 1, nullptr, xe, s));
-        double* c1 = p->phi[l + 1];
-        double* a1 = p->tmp[l + 1];
-        GFD_TRY(vec_zero(c1, p->ops[l + 1]->ng(), s));
-        GFD_TRY(vcycle(p, l + 1, &c1, &a1, p->rho[l + 1], s));
+        GFD_TRY(xfer_dt(p->rst[l], dt, lv.res[l], lv.rho[l + 1], 1, nullptr,
+                        xe, s));
+        void* c1 = lv.phi[l + 1];
+        void* a1 = lv.tmp[l + 1];
+        GFD_TRY(vec_zero_bytes(c1, es * p->ops[l + 1]->ng(), s));
+        GFD_TRY(vcycle(p, lv, sc, l + 1, &c1, &a1, lv.rho[l + 1], s));
         XEpiArgs ce;
         ce.epi = XEPI_SUBFROM;
         ce.base = *cur;
-        GFD_TRY(xfer(p->itp[l], c1, *alt, 1, nullptr, ce, s));
+        GFD_TRY(xfer_dt(p->itp[l], dt, c1, *alt, 1, nullptr, ce, s));
         std::swap(*cur, *alt);
     }
-    GFD_TRY(relax_pp(op, cur, alt, rho, p->post[l], p->w[l], s));
+    GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_post(sc, l),
+                       sched_w(sc, l), s));
+    return OK;
+}
+
+static Schedule ref_schedule(gfd_poisson* p)
+{
+    Schedule sc;
+    sc.coarsest = p->L;
+    sc.pre = p->pre.data();
+    sc.post = p->post.data();
+    sc.w = p->w.data();
+    sc.pre_c = sc.post_c = sc.coarse_sweeps = -1;
+    sc.w0 = sc.wc = -1.0;
+    return sc;
+}
+
+// One V-cycle on (phi, rho) + residual norm into *dn (device).
+static int vcycle64_norm(gfd_poisson* p, double* phi, const double* rho,
+                         double* dn, hipStream_t s)
+{
+    const Schedule sc = ref_schedule(p);
+    void* cur = phi;
+    void* alt = p->lv.tmp[0];
+    GFD_TRY(vcycle(p, p->lv, sc, 0, &cur, &alt, rho, s));
+    if (cur != (void*)phi)
+        GFD_TRY(vec_copy((const double*)cur, phi, p->ops[0]->ng(), s));
+    GFD_TRY(residual_norm(p->ops[0], 0, phi, rho, p->lv.res[0], 1, nullptr,
+                          dn, s));
+    return OK;
+}
+
+static int ensure_stream(gfd_poisson* p)
+{
+    if (!p->gs) {
+        GFD_CHECK_HIP(hipStreamCreateWithFlags(&p->gs, hipStreamNonBlocking));
+        GFD_CHECK_HIP(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming));
+        GFD_CHECK_HIP(hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming));
+    }
+    return OK;
+}
+
+// Capture fn(stream) into an executable graph.  Buffers that the captured
+// work may grow are sized first: nothing allocates during a capture.
+template <typename F>
+static int capture(gfd_poisson* p, hipGraphExec_t* exec, F fn)
+{
+    gfd_operator* op0 = p->ops[0];
+    GFD_TRY(op0->partial.ensure(
+        sizeof(double) * (op_max_blocks(op0, 0, 1) + 1), p->gs));
+    hipGraph_t g = nullptr;
+    GFD_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
+    int st = fn(p->gs);
+    hipError_t e = hipStreamEndCapture(p->gs, &g);
+    if (st != OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        set_error("graph capture failed: %s", hipGetErrorString(e));
+        return EHIP;
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        set_error("graph instantiation failed: %s", hipGetErrorString(e));
+        return EHIP;
+    }
+    return OK;
+}
+
+static int read_scalar(gfd_poisson* p, hipStream_t s, double* h)
+{
+    GFD_CHECK_HIP(hipMemcpyAsync(h, p->scalar.p, sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
     return OK;
 }
 
 static int poisson_iterate(gfd_poisson* p, double* phi, const double* rho,
                            double* error, hipStream_t s)
 {
-    double* cur = phi;
-    double* alt = p->tmp[0];
-    GFD_TRY(vcycle(p, 0, &cur, &alt, rho, s));
-    if (cur != phi)
-        GFD_CHECK_HIP(hipMemcpyAsync(phi, cur, sizeof(double) * p->ops[0]->ng(),
-                                     hipMemcpyDeviceToDevice, s));
-    GFD_TRY(p->scalar.ensure(sizeof(double), s));
-    double* dn = (double*)p->scalar.p;
-    GFD_TRY(residual_norm(p->ops[0], 0, phi, rho, p->res[0], 1, nullptr, dn, s));
+    GFD_TRY(p->scalar.ensure(4 * sizeof(double), s));
+    GFD_TRY(vcycle64_norm(p, phi, rho, (double*)p->scalar.p, s));
     double h = 0.0;
-    GFD_CHECK_HIP(hipMemcpyAsync(&h, dn, sizeof(double), hipMemcpyDeviceToHost,
-                                 s));
-    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    GFD_TRY(read_scalar(p, s, &h));
     *error = h * p->dv;
     return OK;
 }
@@ -859,22 +1027,79 @@ extern "C" int gfd_poisson_iterate(gfd_poisson* p, double* phi,
 
 extern "C" int gfd_add_scalar(double* a, long n, double v, void* stream);
 
+static int remove_mean(gfd_poisson* p, double* phi, hipStream_t s)
+{
+    const long ng = p->ops[0]->ng();
+    GFD_TRY(p->ops[0]->partial.ensure(
+        sizeof(double) * (vec_dot_partials(ng) + 1), s));
+    double* dn = (double*)p->scalar.p;
+    GFD_TRY(vec_dot(phi, nullptr, ng, (double*)p->ops[0]->partial.p, dn, s));
+    double sum = 0.0;
+    GFD_TRY(read_scalar(p, s, &sum));
+    const double ave = sum / (double)ng;
+    return vec_add_scalar(phi, ng, -ave, s);
+}
+
+// Run the solve on the owned stream: phi / rho are copied into the owned
+// level-0 buffers, the captured graph is replayed, phi is copied back.
+static int enter_owned(gfd_poisson* p, double* phi, const double* rho,
+                       hipStream_t s)
+{
+    GFD_TRY(ensure_stream(p));
+    const long ng = p->ops[0]->ng();
+    GFD_TRY(vec_copy(phi, (double*)p->lv.phi[0], ng, s));
+    GFD_TRY(vec_copy(rho, (double*)p->lv.rho[0], ng, s));
+    GFD_CHECK_HIP(hipEventRecord(p->ev_in, s));
+    GFD_CHECK_HIP(hipStreamWaitEvent(p->gs, p->ev_in, 0));
+    return OK;
+}
+
+static int leave_owned(gfd_poisson* p, double* phi, hipStream_t s)
+{
+    GFD_CHECK_HIP(hipEventRecord(p->ev_out, p->gs));
+    GFD_CHECK_HIP(hipStreamWaitEvent(s, p->ev_out, 0));
+    return vec_copy((const double*)p->lv.phi[0], phi, p->ops[0]->ng(), s);
+}
+
 extern "C" int gfd_poisson_solve(gfd_poisson* p, double* phi,
                                  const double* rho, double eps, int maxiter,
-                                 int remove_mean, int* niter, double* error,
-                                 void* stream)
+                                 int remove_mean_flag, int* niter,
+                                 double* error, void* stream)
 {
     if (!p || !phi || !rho || !niter || maxiter < 1) {
         set_error("gfd_poisson_solve: bad arguments");
         return EINVAL_;
     }
     hipStream_t s = (hipStream_t)stream;
+    GFD_TRY(p->scalar.ensure(4 * sizeof(double), s));
+    double* dn = (double*)p->scalar.p;
     int it = 1;
     double err = 0.0;
-    GFD_TRY(poisson_iterate(p, phi, rho, &err, s));
-    while (err > eps && it < maxiter) {
-        it++;
+    if (p->use_graph) {
+        GFD_TRY(enter_owned(p, phi, rho, s));
+        double* phi0 = (double*)p->lv.phi[0];
+        const double* rho0 = (const double*)p->lv.rho[0];
+        if (!p->g64)
+            GFD_TRY(capture(p, &p->g64, [&](hipStream_t cs) {
+                return vcycle64_norm(p, phi0, rho0, dn, cs);
+            }));
+        double h = 0.0;
+        GFD_CHECK_HIP(hipGraphLaunch(p->g64, p->gs));
+        GFD_TRY(read_scalar(p, p->gs, &h));
+        err = h * p->dv;
+        while (err > eps && it < maxiter) {
+            it++;
+            GFD_CHECK_HIP(hipGraphLaunch(p->g64, p->gs));
+            GFD_TRY(read_scalar(p, p->gs, &h));
+            err = h * p->dv;
+        }
+        GFD_TRY(leave_owned(p, phi, s));
+    } else {
         GFD_TRY(poisson_iterate(p, phi, rho, &err, s));
+        while (err > eps && it < maxiter) {
+            it++;
+            GFD_TRY(poisson_iterate(p, phi, rho, &err, s));
+        }
     }
     *niter = it;
     if (error) *error = err;
@@ -883,20 +1108,177 @@ extern "C" int gfd_poisson_solve(gfd_poisson* p, double* phi,
                   maxiter);
         return ECONVERGE;
     }
-    if (remove_mean) {
-        const long ng = p->ops[0]->ng();
-        double sum = 0.0;
-        GFD_TRY(p->ops[0]->partial.ensure(
-            sizeof(double) * (vec_dot_partials(ng) + 1), s));
-        double* dn = (double*)p->scalar.p;
-        GFD_TRY(vec_dot(phi, nullptr, ng, (double*)p->ops[0]->partial.p, dn, s));
-        GFD_CHECK_HIP(hipMemcpyAsync(&sum, dn, sizeof(double),
-                                     hipMemcpyDeviceToHost, s));
-        GFD_CHECK_HIP(hipStreamSynchronize(s));
-        const double ave = sum / (double)ng;
-        GFD_TRY(gfd_add_scalar(phi, ng, -ave, stream));
+    if (remove_mean_flag) GFD_TRY(remove_mean(p, phi, s));
+    return OK;
+}
+
+extern "C" int gfd_poisson_set_graph(gfd_poisson* p, int on)
+{
+    if (!p) {
+        set_error("gfd_poisson_set_graph: null handle");
+        return EINVAL_;
+    }
+    p->use_graph = on != 0;
+    return OK;
+}
+
+// One refinement step of the mixed-precision solve on the owned buffers:
+// r32 = (float) r64; e = 0; `inner` f32 V-cycles on L e = r32;
+// phi -= e; r64 = L phi - rho with its squared norm into *dn.
+static int mixed_step(gfd_poisson* p, const Schedule& sc, int inner,
+                      double* dn, hipStream_t s)
+{
+    const long ng = p->ops[0]->ng();
+    double* phi0 = (double*)p->lv.phi[0];
+    const double* rho0 = (const double*)p->lv.rho[0];
+    double* r64 = (double*)p->lv.res[0];
+    Levels& lv = p->lv32;
+    float* r32 = (float*)lv.rho[0];
+    GFD_TRY(vec_d2f(r64, r32, ng, s));
+    void* cur = lv.phi[0];
+    void* alt = lv.tmp[0];
+    GFD_TRY(vec_zero_bytes(cur, 4 * ng, s));
+    for (int c = 0; c < inner; c++)
+        GFD_TRY(vcycle(p, lv, sc, 0, &cur, &alt, r32, s));
+    GFD_TRY(vec_sub_f(phi0, (const float*)cur, ng, s));
+    return residual_norm(p->ops[0], 0, phi0, rho0, r64, 1, nullptr, dn, s);
+}
+
+extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
+                                       const double* rho, double tol_rel,
+                                       int levels, int pre, int post,
+                                       int coarse_sweeps, double w0,
+                                       double wc, int inner, int max_outer,
+                                       int* n_outer, double* relres,
+                                       void* stream)
+{
+    if (!p || !phi || !rho || !n_outer || max_outer < 1 || inner < 1 ||
+        levels < 1 || pre < 0 || post < 0) {
+        set_error("gfd_poisson_solve_mixed: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int L32 = std::min(levels - 1, p->L);
+    GFD_TRY(alloc_levels(p, p->lv32, 2, L32));
+    GFD_TRY(p->scalar.ensure(4 * sizeof(double), s));
+    double* dn = (double*)p->scalar.p;
+    Schedule sc = ref_schedule(p);
+    sc.coarsest = L32;
+    sc.pre_c = pre;
+    sc.post_c = post;
+    sc.coarse_sweeps = coarse_sweeps;
+    sc.w0 = w0;
+    sc.wc = wc;
+    const int key[6] = {L32, pre, post, coarse_sweeps, inner, 0};
+    bool same = p->g32 && p->g32_w[0] == w0 && p->g32_w[1] == wc;
+    for (int i = 0; i < 6; i++) same = same && p->g32_key[i] == key[i];
+    if (!same && p->g32) {
+        (void)hipGraphExecDestroy(p->g32);
+        p->g32 = nullptr;
+    }
+    GFD_TRY(enter_owned(p, phi, rho, s));
+    const long ng = p->ops[0]->ng();
+    double* phi0 = (double*)p->lv.phi[0];
+    const double* rho0 = (const double*)p->lv.rho[0];
+    // |rho|^2 and the first residual
+    double rn2 = 0.0, err2 = 0.0;
+    GFD_TRY(p->ops[0]->partial.ensure(
+        sizeof(double) * (vec_dot_partials(ng) + 1), p->gs));
+    GFD_TRY(vec_dot(rho0, rho0, ng, (double*)p->ops[0]->partial.p, dn, p->gs));
+    GFD_TRY(read_scalar(p, p->gs, &rn2));
+    GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, p->lv.res[0], 1, nullptr,
+                          dn, p->gs));
+    GFD_TRY(read_scalar(p, p->gs, &err2));
+    const double tol2 = tol_rel * tol_rel * rn2;
+    int outer = 0;
+    while (err2 > tol2 && outer < max_outer) {
+        if (p->use_graph) {
+            if (!p->g32) {
+                GFD_TRY(capture(p, &p->g32, [&](hipStream_t cs) {
+                    return mixed_step(p, sc, inner, dn, cs);
+                }));
+                for (int i = 0; i < 6; i++) p->g32_key[i] = key[i];
+                p->g32_w[0] = w0;
+                p->g32_w[1] = wc;
+            }
+            GFD_CHECK_HIP(hipGraphLaunch(p->g32, p->gs));
+        } else {
+            GFD_TRY(mixed_step(p, sc, inner, dn, p->gs));
+        }
+        GFD_TRY(read_scalar(p, p->gs, &err2));
+        outer++;
+    }
+    GFD_TRY(leave_owned(p, phi, s));
+    *n_outer = outer;
+    if (relres) *relres = rn2 > 0 ? std::sqrt(err2 / rn2) : std::sqrt(err2);
+    if (err2 > tol2) {
+        set_error("mixed-precision Poisson solve did not reach %g in %d "
+                  "refinement steps",
+                  tol_rel, max_outer);
+        return ECONVERGE;
     }
     return OK;
+}
+
+// =================================================================== bands
+
+static BandWork g_band;
+
+extern "C" int gfd_h_apply(gfd_operator* kin, const double* vpot,
+                           const void* psi, void* hpsi, long nbands,
+                           const double* phases, void* stream)
+{
+    if (!kin || !vpot || nbands < 0 || (nbands > 0 && (!psi || !hpsi)) ||
+        psi == hpsi) {
+        set_error("gfd_h_apply: bad arguments");
+        return EINVAL_;
+    }
+    if (nbands == 0) return OK;
+    EpiArgs ea{};
+    ea.c0 = centre_coef(kin);
+    ea.pot = vpot;
+    return op_pass(kin, kin->dtype(), psi, hpsi, nbands,
+                   kin->real ? nullptr : phases, EPI_POT, ea,
+                   (hipStream_t)stream, nullptr);
+}
+
+extern "C" int gfd_band_overlap(const void* psi, int nbands, long npts,
+                                long ld, double dv, void* S, int lower_only,
+                                void* stream)
+{
+    if (!psi || !S || nbands < 0 || npts < 0 || ld < npts) {
+        set_error("gfd_band_overlap: bad arguments");
+        return EINVAL_;
+    }
+    return gram(g_band, (const double2*)psi, ld, (const double2*)psi, ld,
+                nbands, nbands, npts, dv, 0, lower_only, (double2*)S, nbands,
+                (hipStream_t)stream);
+}
+
+extern "C" int gfd_band_matrix(const void* psi, const void* hpsi, int nbands,
+                               long npts, long ld, double dv, void* H,
+                               void* stream)
+{
+    if (!psi || !hpsi || !H || nbands < 0 || npts < 0 || ld < npts) {
+        set_error("gfd_band_matrix: bad arguments");
+        return EINVAL_;
+    }
+    return gram(g_band, (const double2*)psi, ld, (const double2*)hpsi, ld,
+                nbands, nbands, npts, dv, 1, 0, (double2*)H, nbands,
+                (hipStream_t)stream);
+}
+
+extern "C" int gfd_band_cholesky_orth(void* S, int nbands, const void* psi,
+                                      long npts, long ld, void* out,
+                                      void* stream)
+{
+    if (!S || !psi || !out || nbands <= 0 || npts <= 0 || ld < npts ||
+        psi == out) {
+        set_error("gfd_band_cholesky_orth: bad arguments");
+        return EINVAL_;
+    }
+    return cholesky_orth(g_band, (double2*)S, nbands, (const double2*)psi, ld,
+                         npts, (double2*)out, ld, (hipStream_t)stream);
 }
 
 // ==================================================================== misc
@@ -957,6 +1339,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
             return EINVAL_;
         }
         g_force_path = (int)value;
+        return OK;
+    }
+    if (std::strcmp(key, "xfer_passes") == 0) {
+        xfer_force_passes(value != 0);
         return OK;
     }
     if (std::strcmp(key, "xr_tile") == 0) {

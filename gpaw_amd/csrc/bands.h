@@ -1,0 +1,30 @@
+// Band contractions (overlap / subspace matrices, Cholesky orthonormalisation).
+#pragma once
+
+#include "gfd_common.h"
+
+namespace gfd {
+
+// Grow-only device work space of the band calls.  It lives as long as the
+// process (no destructor: the HIP runtime may be gone at static teardown),
+// and calls that overlap on different streams race on it.
+struct BandWork {
+    void* partial = nullptr;
+    size_t partial_bytes = 0;
+    void* blas = nullptr;  // rocblas_handle
+    void* info = nullptr;
+    int ensure_partial(size_t bytes, hipStream_t s);
+};
+
+// out[i][j] = alpha (conj ? conj : id)(sum_k A[i][k] conj(B[j][k])) for
+// i < M, j < N (only 128 x 128 tiles with tile_j <= tile_i if lower_only).
+int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
+         int M, int N, long K, double alpha, int conj, int lower_only,
+         double2* out, long ldo, hipStream_t s);
+
+// Lower Cholesky L of S (row-major, lower part used), C = L^-1 in S, and
+// out_i = sum_j C_ij psi_j for the n rows of psi (X points each).
+int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, hipStream_t s);
+
+}  // namespace gfd

@@ -1,0 +1,282 @@
+// Band contractions of the FD eigensolver step on gfx950 (FP64 MFMA).
+//
+//  * k_gram: G = A B^H for row-major complex A (M x K) and B (N x K) with a
+//    long inner dimension (K = grid points): 128 x 128 output tiles, split
+//    over K into chunks, 512 threads (8 waves, 2 x 4 MFMA tiles of 16 x 16
+//    per wave); A and B stream through LDS in 16-point stages as separate
+//    re / im planes, and every complex 16 x 16 x 4 step is four
+//    v_mfma_f64_16x16x4f64 (re.re + im.im, im.re - re.im).  Chunk partials
+//    are reduced in a fixed order (bit-reproducible).
+//  * the Cholesky factor and its inverse come from rocSOLVER (zpotrf,
+//    ztrtri) on the column-major view of the row-major matrix, and the
+//    rotation Psi <- C Psi is one rocBLAS ztrmm.
+#include "bands.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+
+namespace gfd {
+
+constexpr int GT = 128;  // output tile
+constexpr int GK = 16;   // inner points per LDS stage
+
+typedef double __attribute__((ext_vector_type(4))) d4;
+
+struct GramArgs {
+    const double2* A;
+    const double2* B;
+    long lda, ldb;
+    int M, N;
+    long K;
+    long kchunk;
+    int nchunks;
+    int ntiles;
+    int tcols;           // tile columns (full matrix)
+    int lower;           // tiles (i, j <= i) in row order
+    double2* partial;    // [chunk][tile][GT][GT]
+};
+
+// Output tile t -> (tile row, tile column).
+__device__ inline int2 tile_of(int t, int lower, int tcols)
+{
+    if (!lower) return make_int2(t / tcols, t % tcols);
+    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= t) i++;
+    while (i * (i + 1) / 2 > t) i--;
+    return make_int2(i, t - i * (i + 1) / 2);
+}
+
+__global__ void __launch_bounds__(512) k_gram(GramArgs a)
+{
+    __shared__ double sAr[GK][GT], sAi[GK][GT], sBr[GK][GT], sBi[GK][GT];
+    const int tile = blockIdx.x % a.ntiles;
+    const int chunk = blockIdx.x / a.ntiles;
+    const int2 tt = tile_of(tile, a.lower, a.tcols);
+    const int i0 = tt.x * GT, j0 = tt.y * GT;
+    const long kb = (long)chunk * a.kchunk;
+    const long ke = min(kb + a.kchunk, a.K);
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wr = w >> 1, wc = w & 1;  // 4 x 2 waves: 32 rows x 64 cols
+    const int li = lane & 15, lk = lane >> 4;
+
+    d4 accr[2][4], acci[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            accr[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+
+    // stage loader: 128 rows x 16 points of A and of B, 4 per thread
+    const int lrow = t >> 2, lkq = (t & 3) * 4;
+    for (long ks = kb; ks < ke; ks += GK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long k = ks + lkq + q;
+            double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+            if (k < ke) {
+                if (i0 + lrow < a.M) va = a.A[(long)(i0 + lrow) * a.lda + k];
+                if (j0 + lrow < a.N) vb = a.B[(long)(j0 + lrow) * a.ldb + k];
+            }
+            sAr[lkq + q][lrow] = va.x;
+            sAi[lkq + q][lrow] = va.y;
+            sBr[lkq + q][lrow] = vb.x;
+            sBi[lkq + q][lrow] = vb.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK / 4; kk++) {
+            const int kl = kk * 4 + lk;
+            double ar[2], ai[2], br[4], bi[4], nbi[4];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                ar[r] = sAr[kl][wr * 32 + r * 16 + li];
+                ai[r] = sAi[kl][wr * 32 + r * 16 + li];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                br[c] = sBr[kl][wc * 64 + c * 16 + li];
+                bi[c] = sBi[kl][wc * 64 + c * 16 + li];
+                nbi[c] = -bi[c];
+            }
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ar[r], br[c], accr[r][c], 0, 0, 0);
+                    accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ai[r], bi[c], accr[r][c], 0, 0, 0);
+                    acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ai[r], br[c], acci[r][c], 0, 0, 0);
+                    acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ar[r], nbi[c], acci[r][c], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // D register q of lane l holds row l / 16 + 4 q, column l % 16
+    double2* out = a.partial + ((long)chunk * a.ntiles + tile) * GT * GT;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = wr * 32 + r * 16 + lk + 4 * q;
+                const int col = wc * 64 + c * 16 + li;
+                out[row * GT + col] = make_double2(accr[r][c][q],
+                                                   acci[r][c][q]);
+            }
+}
+
+// out[i][j] = alpha (conj?) sum_chunks partial, for every computed tile
+__global__ void __launch_bounds__(256)
+k_gram_reduce(const double2* __restrict__ partial, int nchunks, int ntiles,
+              int lower, int tcols, int M, int N, double alpha, int conj,
+              double2* __restrict__ out, long ldo)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long per = (long)GT * GT;
+    if (e >= per * ntiles) return;
+    const int tile = (int)(e / per);
+    const int rc = (int)(e % per);
+    const int2 tt = tile_of(tile, lower, tcols);
+    const int row = tt.x * GT + rc / GT;
+    const int col = tt.y * GT + rc % GT;
+    if (row >= M || col >= N) return;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < nchunks; c++) {
+        const double2 v = partial[((long)c * ntiles + tile) * per + rc];
+        sr += v.x;
+        si += v.y;
+    }
+    out[(long)row * ldo + col] =
+        make_double2(alpha * sr, conj ? -alpha * si : alpha * si);
+}
+
+int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
+         int M, int N, long K, double alpha, int conj, int lower_only,
+         double2* out, long ldo, hipStream_t s)
+{
+    const int TM = (M + GT - 1) / GT, TN = (N + GT - 1) / GT;
+    if (lower_only && TM != TN) {
+        set_error("gram: lower_only needs a square matrix");
+        return EINVAL_;
+    }
+    const int nt = lower_only ? TM * (TM + 1) / 2 : TM * TN;
+    if (nt == 0) return OK;
+    if (K <= 0) {
+        set_error("gram: no points");
+        return EINVAL_;
+    }
+    // chunks: about 9 blocks per CU (256 CUs), at least 4 stages each
+    long chunks = std::max(1L, (long)(2304 + nt - 1) / nt);
+    chunks = std::min(chunks, std::max(1L, K / (4 * GK)));
+    long kchunk = (K + chunks - 1) / chunks;
+    kchunk = (kchunk + GK - 1) / GK * GK;
+    chunks = (K + kchunk - 1) / kchunk;
+    const size_t pbytes = sizeof(double2) * (size_t)GT * GT * nt * chunks;
+    GFD_TRY(w.ensure_partial(pbytes, s));
+    GramArgs g;
+    g.A = A;
+    g.B = B;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.kchunk = kchunk;
+    g.nchunks = (int)chunks;
+    g.ntiles = nt;
+    g.tcols = TN;
+    g.lower = lower_only;
+    g.partial = (double2*)w.partial;
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)(nt * chunks)), dim3(512), 0, s,
+                       g);
+    GFD_LAUNCH_CHECK();
+    const long total = (long)GT * GT * nt;
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((total + 255) / 256)),
+                       dim3(256), 0, s, (const double2*)w.partial,
+                       (int)chunks, nt, lower_only, TN, M, N, alpha, conj,
+                       out, ldo);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int BandWork::ensure_partial(size_t bytes, hipStream_t s)
+{
+    if (bytes <= partial_bytes) return OK;
+    if (partial) {
+        GFD_CHECK_HIP(hipStreamSynchronize(s));
+        GFD_CHECK_HIP(hipFree(partial));
+        partial = nullptr;
+        partial_bytes = 0;
+    }
+    if (hipMalloc(&partial, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        partial = nullptr;
+        set_error("hipMalloc of %zu bytes failed", bytes);
+        return ENOMEM_;
+    }
+    partial_bytes = bytes;
+    return OK;
+}
+
+static int blas_ok(rocblas_status st, const char* what)
+{
+    if (st != rocblas_status_success) {
+        set_error("%s failed: %s", what, rocblas_status_to_string(st));
+        return EHIP;
+    }
+    return OK;
+}
+
+int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, hipStream_t s)
+{
+    if (!w.blas) {
+        rocblas_handle h;
+        GFD_TRY(blas_ok(rocblas_create_handle(&h), "rocblas_create_handle"));
+        w.blas = h;
+        GFD_CHECK_HIP(hipMalloc(&w.info, sizeof(int)));
+    }
+    rocblas_handle h = (rocblas_handle)w.blas;
+    GFD_TRY(blas_ok(rocblas_set_stream(h, s), "rocblas_set_stream"));
+    // column-major view of the row-major lower triangle = upper triangle
+    GFD_TRY(blas_ok(rocsolver_zpotrf(h, rocblas_fill_upper, n,
+                                     (rocblas_double_complex*)S, n,
+                                     (rocblas_int*)w.info),
+                    "rocsolver_zpotrf"));
+    int info = 0;
+    GFD_CHECK_HIP(hipMemcpyAsync(&info, w.info, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    if (info != 0) {
+        set_error("overlap matrix is not positive definite (zpotrf info %d)",
+                  info);
+        return EINVAL_;
+    }
+    GFD_TRY(blas_ok(rocsolver_ztrtri(h, rocblas_fill_upper,
+                                     rocblas_diagonal_non_unit, n,
+                                     (rocblas_double_complex*)S, n,
+                                     (rocblas_int*)w.info),
+                    "rocsolver_ztrtri"));
+    // Psi^T (X x n, column-major) times the upper factor: psi_i <- C_ij psi_j
+    const rocblas_double_complex one = {1.0, 0.0};
+    GFD_TRY(blas_ok(rocblas_ztrmm_64(h, rocblas_side_right, rocblas_fill_upper,
+                                     rocblas_operation_none,
+                                     rocblas_diagonal_non_unit, X, n, &one,
+                                     (const rocblas_double_complex*)S, n,
+                                     (const rocblas_double_complex*)psi,
+                                     ldpsi, (rocblas_double_complex*)out,
+                                     ldout),
+                    "rocblas_ztrmm"));
+    return OK;
+}
+
+}  // namespace gfd

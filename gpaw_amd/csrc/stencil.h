@@ -16,6 +16,8 @@ enum Epi : int {
     EPI_RELAX = 1,   // out = (1 - w) v + w (aux - acc) / c0        (Jacobi)
     EPI_RESID = 2,   // out = (c0 v + acc) - aux          [+ sum out^2]
     EPI_UPDATE = 3,  // out = +-(v - s ((c0 v + acc) - aux))
+    EPI_POT = 4,     // out = (c0 v + acc) + pot v, pot real per grid point
+                     // (shared by the grids of a batch): H psi
 };
 
 struct EpiArgs {
@@ -25,6 +27,7 @@ struct EpiArgs {
     double s;          // UPDATE step
     int negate;        // UPDATE: store -(...)
     double* partial;   // RESID: per-block sum of |out|^2, or 0
+    const double* pot; // POT: real potential of one grid
 };
 
 // Axis-only (central-difference) stencil in per-axis form.

@@ -299,6 +299,7 @@ struct GenArgs {
     int negate;
     BC bc;
     double* partial;
+    const double* pot;
 };
 
 constexpr int GEN_BLOCK = 256;
@@ -336,8 +337,14 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
             acc = fmac(acc, (S)a.coef[c], v);
         }
         const T v = g[((long)i0 * a.n1 + i1) * a.n2 + i2];
-        a.out[gid] = epilogue<T, EPI>(acc, v, a.aux, gid, a.c0, a.w, a.s,
-                                      a.negate, nrm);
+        if constexpr (EPI == EPI_POT) {
+            typedef typename ScalarOf<T>::type S;
+            a.out[gid] = fmac(fmac(acc, (S)a.c0, v), (S)a.pot[gid - b * a.ng],
+                              v);
+        } else {
+            a.out[gid] = epilogue<T, EPI>(acc, v, a.aux, gid, a.c0, a.w, a.s,
+                                          a.negate, nrm);
+        }
     }
     if (EPI == EPI_RESID && a.partial)
         block_sum_store(nrm, a.partial, blockIdx.x);
@@ -607,6 +614,7 @@ static int launch_gen_t(const GeneralStencil& st, const int n[3],
     a.negate = ea.negate;
     a.bc = bc;
     a.partial = ea.partial;
+    a.pot = ea.pot;
     const long nb = (a.total + GEN_BLOCK - 1) / GEN_BLOCK;
     if (nb <= 0) return OK;
     if (nb > 0x7fffffffL) {
@@ -631,9 +639,11 @@ int launch_general_stencil(const GeneralStencil& st, int dtype,
         case EPI_RELAX: return launch_gen_t<double, EPI_RELAX>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RESID: return launch_gen_t<double, EPI_RESID>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_UPDATE: return launch_gen_t<double, EPI_UPDATE>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_POT: return launch_gen_t<double, EPI_POT>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         }
     } else if (dtype == 1) {
         switch (epi) {
+        case EPI_POT: return launch_gen_t<double2, EPI_POT>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_PLAIN: return launch_gen_t<double2, EPI_PLAIN>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RESID: return launch_gen_t<double2, EPI_RESID>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_UPDATE: return launch_gen_t<double2, EPI_UPDATE>(st, n, bc, in, out, nin, ea, stream, nblocks_out);

@@ -48,10 +48,11 @@ struct XrArgs {
     double cm[3][MAXR_FAST];
     BC bc;
     double* partial;
+    const double* pot;
 };
 
 constexpr int XR_MAXT = 512;
-constexpr int XR_HMAX = 4;
+constexpr int XR_HROWS = 3;  // halo rows loaded per thread per plane
 
 template <typename T, int PZ>
 __device__ inline Chunk<T, PZ> zero_chunk()
@@ -120,18 +121,29 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
     // own rows (never wrapped: TY divides n1) and the LDS row of row 0
     const int own0 = (y0 + sg * PY) * n2 + z0;
     const int lrow0 = (sg * PY + R) * n2;
-    // halo row of this thread (segments 0 .. 2R-1 load one row each)
-    const bool hload = (a.hasy || a.hasz) && sg < 2 * R;
-    int hoff = 0, hlds = 0;
-    bool hok = false, hph = false;
-    double2 hf = make_double2(1.0, 0.0);
-    if (hload) {
-        const int row = sg < R ? sg - R : TY + sg - R;
-        int yw = y0 + row;
-        hok = wrap_full(yw, n1, bc.periodic[1], bc.phase[1][0],
-                        bc.phase[1][1], hf, hph);
-        hoff = yw * n2 + z0;
-        hlds = (row + R) * n2 + z0;
+    // halo rows of the tile: segment sg loads rows sg, sg + NSEG, ... < 2R
+    const int NSEG = blockDim.y;
+    const bool anyhalo = a.hasy || a.hasz;
+    bool hload[XR_HROWS], hok[XR_HROWS], hph[XR_HROWS];
+    int hoff[XR_HROWS], hlds[XR_HROWS];
+    double2 hf[XR_HROWS];
+#pragma unroll
+    for (int m = 0; m < XR_HROWS; m++) {
+        const int hr = sg + m * NSEG;
+        hload[m] = anyhalo && hr < 2 * R;
+        hok[m] = false;
+        hph[m] = false;
+        hoff[m] = 0;
+        hlds[m] = 0;
+        hf[m] = make_double2(1.0, 0.0);
+        if (hload[m]) {
+            const int row = hr < R ? hr - R : TY + hr - R;
+            int yw = y0 + row;
+            hok[m] = wrap_full(yw, n1, bc.periodic[1], bc.phase[1][0],
+                               bc.phase[1][1], hf[m], hph[m]);
+            hoff[m] = yw * n2 + z0;
+            hlds[m] = (row + R) * n2 + z0;
+        }
     }
 
     S cpx[R], cmx[R], cpy[R], cmy[R], cpz[R], cmz[R];
@@ -172,17 +184,21 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                 if (phx) dst[j] = phase_chunk<T, PZ>(dst[j], fx);
         }
     };
-    auto load_halo = [&](int x, C& dst) {
+    auto load_halo = [&](int x, C* dst) {
         double2 fx;
         bool phx;
         const long base = plane_of(x, fx, phx);
-        dst = zero_chunk<T, PZ>();
-        if (base >= 0 && hok) {
-            dst = *reinterpret_cast<const C*>(gin + base + hoff);
-            if constexpr (IsCplx<T>::value) {
-                if (phx || hph) {
-                    const double2 f = phx ? (hph ? cmul(hf, fx) : fx) : hf;
-                    dst = phase_chunk<T, PZ>(dst, f);
+#pragma unroll
+        for (int m = 0; m < XR_HROWS; m++) {
+            dst[m] = zero_chunk<T, PZ>();
+            if (hload[m] && base >= 0 && hok[m]) {
+                dst[m] = *reinterpret_cast<const C*>(gin + base + hoff[m]);
+                if constexpr (IsCplx<T>::value) {
+                    if (phx || hph[m]) {
+                        const double2 f =
+                            phx ? (hph[m] ? cmul(hf[m], fx) : fx) : hf[m];
+                        dst[m] = phase_chunk<T, PZ>(dst[m], f);
+                    }
                 }
             }
         }
@@ -192,10 +208,10 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
     };
 
     C q[U][PY];
-    C hv;
+    C hv[XR_HROWS];
 #pragma unroll
     for (int k = -R; k <= R; k++) load_own(xs + k, q[(k + U) % U]);
-    if (hload) load_halo(xs, hv);
+    if (anyhalo) load_halo(xs, hv);
 
     double nrm = 0.0;
     for (int xg = xs; xg < xe; xg += U) {
@@ -209,12 +225,15 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                     for (int j = 0; j < PY; j++)
                         *reinterpret_cast<C*>(buf + lrow0 + j * n2 + z0) =
                             q[u][j];
-                    if (hload) *reinterpret_cast<C*>(buf + hlds) = hv;
+#pragma unroll
+                    for (int m = 0; m < XR_HROWS; m++)
+                        if (hload[m])
+                            *reinterpret_cast<C*>(buf + hlds[m]) = hv[m];
                 }
                 // halo first: its wait at the next step leaves the own-plane
                 // loads issued after it in flight
                 if (x + 1 < xe) {
-                    if (hload) load_halo(x + 1, hv);
+                    if (anyhalo) load_halo(x + 1, hv);
                     load_own(x + R + 1, q[(u + R + 1) % U]);
                 }
                 if (a.hasy || a.hasz) __syncthreads();
@@ -282,14 +301,24 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                     }
                     const long idx = obase + (long)j * n2;
                     C av = zero_chunk<T, PZ>();
-                    if constexpr (EPI != EPI_PLAIN)
+                    if constexpr (EPI != EPI_PLAIN && EPI != EPI_POT)
                         av = *reinterpret_cast<const C*>(a.aux + idx);
                     C r;
+                    if constexpr (EPI == EPI_POT) {
+                        // + V psi, V real and shared by all grids
+                        const double* pv =
+                            a.pot + (long)x * plane + own0 + j * n2;
 #pragma unroll
-                    for (int p = 0; p < PZ; p++)
-                        r.v[p] = epi_apply<T, EPI>(acc[p], q[u][j].v[p],
-                                                   av.v[p], a.c0, a.w, a.s,
-                                                   a.negate, nrm);
+                        for (int p = 0; p < PZ; p++)
+                            r.v[p] = fmac(fmac(acc[p], (S)a.c0, q[u][j].v[p]),
+                                          (S)pv[p], q[u][j].v[p]);
+                    } else {
+#pragma unroll
+                        for (int p = 0; p < PZ; p++)
+                            r.v[p] = epi_apply<T, EPI>(acc[p], q[u][j].v[p],
+                                                       av.v[p], a.c0, a.w,
+                                                       a.s, a.negate, nrm);
+                    }
                     *reinterpret_cast<C*>(a.out + idx) = r;
                 }
             }
@@ -313,7 +342,7 @@ static bool xr_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
     const int NT = TZT * t.nseg;
     const int TY = t.nseg * t.py;
     const long lds = 2L * (TY + 2 * R) * n[2] * xr_es(dtype);
-    return NT <= XR_MAXT && t.nseg >= 2 * R && n[1] % TY == 0 &&
+    return NT <= XR_MAXT && t.nseg * XR_HROWS >= 2 * R && n[1] % TY == 0 &&
            n[2] >= ((R + PZ - 1) / PZ) * PZ && R <= n[0] && R <= n[1] &&
            t.xchunk >= 1 && lds <= 150 * 1024;
 }
@@ -345,15 +374,16 @@ XmTile xr_default_tile(int R, int dtype, const int n[3], long nin)
     // rows per thread: 2R+2 queue slots of PY x 16 bytes must stay well
     // inside 256 VGPRs
     const int pymax = R >= 3 ? 2 : 4;
-    // the smallest segment count >= 2R (one halo row per thread) whose tile
-    // height divides n1, preferring 2 rows per thread and ~256-384 threads
+    // a segment count whose tile height divides n1, preferring one halo row
+    // per thread (s >= 2R), more rows per thread and ~256-384 threads
     int best_score = -1;
     for (int py = pymax; py >= 1; py >>= 1)
-        for (int s = 2 * R; s * TZT <= XR_MAXT; s++) {
+        for (int s = (2 * R + XR_HROWS - 1) / XR_HROWS; s * TZT <= XR_MAXT;
+             s++) {
             if (n[1] % (s * py) != 0) continue;
             const int nt = s * TZT;
             const int score = (nt >= 192 ? 100 : nt / 2) + 10 * py -
-                              (nt > 384 ? 20 : 0);
+                              (nt > 384 ? 20 : 0) + (s >= 2 * R ? 30 : 0);
             if (score > best_score) {
                 best_score = score;
                 t.nseg = s;
@@ -429,6 +459,7 @@ static int launch_xr_t(const AxisStencil& st, const XmTile& tl,
         }
     a.bc = bc;
     a.partial = ea.partial;
+    a.pot = ea.pot;
     const int TZT = n[2] / PZ;
     const size_t shm =
         (st.has[1] || st.has[2]) ? 2 * (size_t)(TY + 2 * R) * n[2] * sizeof(T)
@@ -501,9 +532,11 @@ int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
         case EPI_RELAX: return launch_xr_r<double, EPI_RELAX>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RESID: return launch_xr_r<double, EPI_RESID>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_UPDATE: return launch_xr_r<double, EPI_UPDATE>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_POT: return launch_xr_r<double, EPI_POT>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         }
     } else if (dtype == 1) {
         switch (epi) {
+        case EPI_POT: return launch_xr_r<double2, EPI_POT>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_PLAIN: return launch_xr_r<double2, EPI_PLAIN>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RESID: return launch_xr_r<double2, EPI_RESID>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_UPDATE: return launch_xr_r<double2, EPI_UPDATE>(st, tl, n, bc, in, out, nin, ea, stream, nblocks_out);

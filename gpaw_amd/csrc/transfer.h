@@ -37,6 +37,9 @@ struct XferPlan {
     XferAxis ax[3];
 };
 
+// K = 2 transfers run as one fused kernel unless forced to the 3 passes.
+void xfer_force_passes(bool on);
+
 // Scratch (in elements of the data type) needed for `nin` grids.
 long xfer_scratch_elems(const XferPlan& p, long nin);
 

