@@ -7,10 +7,15 @@ roofline.  One step = one Laplace apply over all 256 bands.
 
 Also reported in ``stages`` (same run): Gradient-x nn=3 on the same bands,
 restrict / interpolate nn=3 on 4 densities 96^3 <-> 48^3, the FP64 multigrid
-Poisson solve of the 48^3 four-Gaussian problem (eps 2e-10) and the kinetic
-preconditioner on 32 complex bands.
+Poisson solve of the 48^3 four-Gaussian problem (eps 2e-10), the kinetic
+preconditioner on 32 complex bands, the 192^3 Poisson time to a 1e-10
+relative residual (all-FP64 V-cycles against FP32 V-cycles inside FP64
+refinement) and the FD eigensolver step on 1024 complex bands of 128^3 at
+4 k-points (H apply, preconditioner, FP64 MFMA overlap, Cholesky
+orthonormalisation).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--no-stages] [--no-eigen]
 """
 from __future__ import annotations
 
@@ -43,6 +48,11 @@ def parse():
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--no-stages', action='store_true')
     p.add_argument('--bands', type=int, default=256)
+    p.add_argument('--no-eigen', action='store_true')
+    p.add_argument('--pmc', action='store_true',
+                   help='counter runs: 3 headline applies and 3 copies of '
+                        'the same 3.62 GB (calibration), no timing line')
+    p.add_argument('--eigen-bands', type=int, default=1024)
     return p.parse_args()
 
 
@@ -75,6 +85,133 @@ def traffic_from_profiles(name):
         if line.startswith('traffic_bytes_per_launch:'):
             return float(line.split(':', 1)[1])
     return None
+
+
+def stage_poisson_192(dev, tol=1e-10, levels=5, inner=2, coarse=24):
+    """Time to |L phi - rho| / |rho| <= tol from phi = 0 on the 192^3
+    problem: all-FP64 V-cycles (reference schedule, every level) against
+    FP32 V-cycles (2 pre / 2 post Jacobi sweeps, `levels` levels, `coarse`
+    sweeps on the coarsest, `inner` cycles per step) inside FP64 residual
+    refinement."""
+    import torch
+    from gpaw_amd import workloads
+    from gpaw_amd.poisson import FDPoissonSolver
+    gd, rho = workloads.poisson_192()
+    drho = torch.from_numpy(rho).to(dev)
+    rn2 = float((rho**2).sum())
+    s = FDPoissonSolver(nn=3, eps=tol**2 * rn2 * gd.dv, maxiter=1000)
+    s.set_grid_descriptor(gd)
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, r
+
+    phi64 = torch.zeros_like(drho)
+    s.solve_neutral(phi64, drho)              # capture + warm
+    t64 = []
+    for _ in range(3):
+        phi64.zero_()
+        ms, n64 = wall(lambda: s.solve_neutral(phi64, drho))
+        t64.append(ms)
+    rel64 = (s.last_error / (rn2 * gd.dv))**0.5
+    phi = torch.zeros_like(drho)
+    kw = dict(levels=levels, pre=2, post=2, inner=inner, coarse_sweeps=coarse)
+    s.solve_mixed(phi, drho, tol, **kw)
+    tmx = []
+    for _ in range(3):
+        phi.zero_()
+        ms, (nmx, relmx) = wall(lambda: s.solve_mixed(phi, drho, tol, **kw))
+        tmx.append(ms)
+    a = phi - phi.mean()
+    b = phi64 - phi64.mean()
+    err = float((a - b).abs().max() / b.abs().max())
+    return {'fp64_ms': round(min(t64), 3), 'fp64_vcycles': n64,
+            'fp64_relres': rel64, 'mixed_ms': round(min(tmx), 3),
+            'mixed_refinements': nmx, 'mixed_relres': relmx,
+            'speedup_mixed_vs_fp64': round(min(t64) / min(tmx), 3),
+            'mixed_err_vs_fp64': err,
+            'levels_fp64': s.levels + 1, 'levels_mixed': levels,
+            'inner_cycles': inner, 'coarse_sweeps': coarse}
+
+
+def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
+    """One FD eigensolver step per k-point (4 Bloch k-points) on nbands
+    complex bands of 128^3: H psi = -1/2 lap(nn=3) psi + V psi, the
+    multigrid kinetic preconditioner on H psi, S = dv psi psi^H (FP64
+    MFMA, lower tiles) and psi <- chol(S)^-1 psi."""
+    import torch
+    from gpaw_amd import workloads
+    from gpaw_amd.bands import apply_hamiltonian, orthonormalize
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import Preconditioner
+    gd = workloads.eigen_128_gd()
+    phases = workloads.eigen_128_kpoints()
+    vt = torch.from_numpy(workloads.eigen_128_potential(gd)).to(dev)
+    kin = Laplace(gd, -0.5, 3, complex)
+    P = Preconditioner(gd, kin, complex, block=block)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1006)
+    shape = (nbands, 128, 128, 128)
+    psi = torch.randn(shape, dtype=torch.complex128, device=dev,
+                      generator=g)
+    hpsi = torch.empty_like(psi)
+    pre = torch.empty_like(psi)
+    S = torch.zeros((nbands, nbands), dtype=torch.complex128, device=dev)
+
+    class Kpt:
+        phase_cd = None
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    acc = np.zeros(4)
+
+    def step(record):
+        for k in range(len(phases)):
+            Kpt.phase_cd = phases[k]
+            ev[0].record()
+            apply_hamiltonian(kin, vt, psi, hpsi, phases[k])
+            ev[1].record()
+            P(hpsi, Kpt, out=pre)
+            ev[2].record()
+            # orthonormalised bands land in hpsi (free after the
+            # preconditioner); S is computed inside orthonormalize
+            orthonormalize(psi, gd.dv, out=hpsi, S=S)
+            ev[3].record()
+            torch.cuda.synchronize()
+            if record:
+                acc[0] += ev[0].elapsed_time(ev[1])
+                acc[1] += ev[1].elapsed_time(ev[2])
+                acc[2] += ev[2].elapsed_time(ev[3])
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    # the overlap alone, for its MFMA rate
+    from gpaw_amd.bands import overlap
+    ms_s, _ = timed(lambda: overlap(psi, gd.dv, True, S), 3, 1)
+    X = 128**3
+    nk = len(phases)
+    herk_flop = nbands * (nbands + 1) / 2 * X * 8
+    h_bytes = nbands * X * 32
+    per = acc / (steps * nk)
+    return {'ms_per_step_4k': round(wall, 2),
+            'h_apply_ms_per_k': round(per[0], 3),
+            'h_apply_hbm_frac': round(h_bytes / per[0] / 1e6 / HBM_PEAK_GBS,
+                                      4),
+            'precond_ms_per_k': round(per[1], 3),
+            'overlap_orth_ms_per_k': round(per[2], 3),
+            'overlap_ms': round(ms_s, 3),
+            'overlap_mfma_tflops': round(herk_flop / ms_s / 1e9, 2),
+            'overlap_mfma_frac': round(herk_flop / ms_s / 1e9 / 78.6, 4),
+            'bands': nbands, 'kpoints': nk, 'grid': [128, 128, 128],
+            'note': 'preconditioner applied to H psi; FP64 path'}
 
 
 def cpu_baseline(gd, op_coefs, op_offsets, nbands_sample):
@@ -112,7 +249,7 @@ def cpu_baseline(gd, op_coefs, op_offsets, nbands_sample):
         run()
         reps += 1
         el = time.perf_counter() - t0
-        if el > 8.0 or reps >= 50:
+        if el > 12.0 or reps >= 200:
             break
     pts = reps * a.size
     return {'value': round(pts / el / 1e9, 4), 'unit': 'GPts/s',
@@ -144,6 +281,18 @@ def main():
     a = torch.rand((nb, 96, 96, 96), dtype=torch.float64, device=dev,
                    generator=g) * 2 - 1
     b = torch.empty_like(a)
+    if args.pmc:
+        for _ in range(3):
+            lap.apply(a, b)
+        torch.cuda.synchronize()
+        lib = _lib.load()
+        nbytes = a.numel() * 8
+        for _ in range(3):
+            _lib.check(lib.gfd_stream_copy(a.data_ptr(), b.data_ptr(),
+                                           nbytes, torch.cuda.current_stream().cuda_stream), "copy")
+        torch.cuda.synchronize()
+        log(f'[bench] pmc mode: 3 Laplace applies, 3 copies of {nbytes} B')
+        return
     ms, wall = timed(lambda: lap.apply(a, b), args.steps, args.warmup)
     pts = nb * 96**3
     value = pts / (ms * 1e-3) / 1e9
@@ -168,6 +317,14 @@ def main():
                             parity[f'band{nb - 1}_relerr']) < 1e-13
                         and parity['max_band_sum_over_l1'] < 1e-12)
     log('[bench] parity', parity)
+    # measured streaming ceiling: a 16-byte copy of the same bytes (after
+    # the parity check: it overwrites b)
+    lib = _lib.load()
+    ms_copy, _ = timed(lambda: _lib.check(lib.gfd_stream_copy(
+        a.data_ptr(), b.data_ptr(), a.numel() * 8, torch.cuda.current_stream(
+        ).cuda_stream), 'copy'), 10, 2)
+    copy_gbs = alg_bytes / (ms_copy * 1e-3) / 1e9
+    log(f'[bench] copy ceiling {copy_gbs:.0f} GB/s')
 
     stages = {}
     if not args.no_stages:
@@ -233,6 +390,16 @@ def main():
             'ms': round(ms_p, 3),
             'GPts/s': round(32 * 128**3 / ms_p / 1e6, 1)}
         log('[bench] stages', stages)
+        torch.cuda.empty_cache()
+        stages['poisson_192_time_to_1e-10'] = stage_poisson_192(dev)
+        log('[bench] poisson 192', stages['poisson_192_time_to_1e-10'])
+        del res, out, P
+        torch.cuda.empty_cache()
+        if not args.no_eigen:
+            stages['eigen_step_c128_128'] = stage_eigen_128(
+                dev, args.eigen_bands)
+            log('[bench] eigen', stages['eigen_step_c128_128'])
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
@@ -263,6 +430,10 @@ def main():
                      'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': traffic},
         'cpu_baseline': cpu,
+        'copy_ceiling': {'GB/s': round(copy_gbs, 1),
+                         'laplace_frac_of_copy': round(achieved / copy_gbs,
+                                                       4),
+                         'note': '16-byte device copy of the same 3.62 GB'},
         'parity': parity,
         'stages': stages,
     }

@@ -16,8 +16,10 @@ LIB = os.path.join(HERE, 'libgpaw_fd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = 'gfx950'
 
-SOURCES = ['api.hip', 'stencil.hip', 'stencil_rows.hip', 'transfer.hip',
-           'vector.hip', 'bands.hip']
+# longest compiles first (the pool runs them in this order)
+SOURCES = ['rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
+           'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
+           'stencil_rows.hip', 'vector.hip', 'bands.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',

@@ -43,6 +43,7 @@ SIGNATURES = {
     'gfd_transformer_free': (None, [_vp]),
     'gfd_transformer_apply': (_i, [_vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_fd_precond': (_i, [_vp] * 7 + [_vp] * 8 + [_vp, _l, _d, _dp, _vp]),
+    'gfd_fd_precond32': (_i, [_vp] * 7 + [_vp] * 8 + [_vp, _l, _d, _dp, _vp]),
     'gfd_poisson_new': (_i, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                              ctypes.POINTER(_vp), _ip, _ip, _dp, _i, _d,
                              ctypes.POINTER(_vp)]),
@@ -52,6 +53,20 @@ SIGNATURES = {
     'gfd_poisson_set_graph': (_i, [_vp, _i]),
     'gfd_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _i, _d,
                                      _d, _i, _i, _ip, _dp, _vp]),
+    'gfd_batch_poisson_new': (_i, [_i, _i, ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                   _dp, ctypes.POINTER(_vp)]),
+    'gfd_batch_poisson_free': (None, [_vp]),
+    'gfd_batch_poisson_level_points': (_l, [_vp, _i]),
+    'gfd_batch_poisson_tune': (_i, [_vp, _vp]),
+    'gfd_batch_poisson_tiles': (_i, [_vp, _i, _ip, _ip, _ip,
+                                     ctypes.POINTER(_l)]),
+    'gfd_batch_poisson_apply': (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    'gfd_batch_poisson_transfer': (_i, [_vp, _i, _i, _i, _vp, _vp, _vp,
+                                        _vp]),
+    'gfd_batch_poisson_solve_mixed': (_i, [_vp, ctypes.POINTER(_vp),
+                                           ctypes.POINTER(_vp), _d, _i, _i,
+                                           _i, _i, _i, _ip, _dp, _vp]),
     'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
     'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),

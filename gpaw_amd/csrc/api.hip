@@ -16,6 +16,7 @@
 #include "transfer.h"
 #include "vector.h"
 #include "bands.h"
+#include "handles.h"
 
 namespace gfd {
 
@@ -34,38 +35,7 @@ void set_error(const char* fmt, ...)
 const char* last_error() { return g_err.c_str(); }
 
 // 0 auto, 1 general kernel, 2 tiled x-march kernel, 3 full-row kernel
-static int g_force_path = 0;
-
-// Grow-only device buffer.  Growing waits for the stream first, so a buffer
-// still read by queued kernels is never freed under them.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t want, hipStream_t s)
-    {
-        if (want <= bytes) return OK;
-        if (p) {
-            GFD_CHECK_HIP(hipStreamSynchronize(s));
-            GFD_CHECK_HIP(hipFree(p));
-            p = nullptr;
-            bytes = 0;
-        }
-        if (hipMalloc(&p, want) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            set_error("hipMalloc of %zu bytes failed", want);
-            return ENOMEM_;
-        }
-        bytes = want;
-        return OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
+int g_force_path = 0;
 
 static BC make_bc(const int periodic[3], const double* phases)
 {
@@ -100,38 +70,6 @@ static int periodic_from_neighbors(const long nb[6], int periodic[3])
 using namespace gfd;
 
 // ====================================================================== ops
-
-struct gfd_operator {
-    int P = 0;
-    std::vector<double> coef;
-    std::vector<int> off;  // P x 3
-    int n[3] = {0, 0, 0};
-    int range = 0;
-    int periodic[3] = {0, 0, 0};
-    int real = 1;
-    double diag = 0.0;      // get_diagonal_element (last zero offset)
-    int centre = -1;        // first zero offset
-    bool axis_ok = false;
-    AxisStencil ax{};
-    double* d_coef = nullptr;
-    int* d_off = nullptr;
-    DevBuf scratch;         // relax ping-pong / in-place copies
-    DevBuf partial;         // residual-norm partial sums
-    DevBuf scalar;          // one device double
-    long ng() const { return (long)n[0] * n[1] * n[2]; }
-    int dtype() const { return real ? 0 : 1; }
-    GeneralStencil gen(double c0) const
-    {
-        GeneralStencil g;
-        g.P = P;
-        g.centre = centre;
-        g.c0 = c0;
-        g.coef = d_coef;
-        g.off = d_off;
-        return g;
-    }
-    bool use_fast() const { return axis_ok && g_force_path != 1; }
-};
 
 extern "C" const char* gfd_last_error(void) { return gfd::last_error(); }
 
@@ -298,7 +236,13 @@ static int op_path(gfd_operator* op, int dtype, long nin, const void* in,
                      (aux == nullptr || aligned16(aux)) &&
                      row_stencil_fits(op->ax, dtype, op->n, nin);
     if (row) return 3;
-    if (epi != EPI_POT && (g_force_path == 0 || g_force_path == 2) &&
+    // epilogues the tiled x-march kernel instantiates per data type
+    const bool xm_has =
+        (dtype == 0 && epi <= EPI_UPDATE) ||
+        (dtype == 1 && (epi == EPI_PLAIN || epi == EPI_RESID ||
+                        epi == EPI_UPDATE)) ||
+        (dtype == 2 && epi <= EPI_RESID);
+    if (xm_has && (g_force_path == 0 || g_force_path == 2) &&
         axis_stencil_fits(op->ax, dtype, op->n, nin))
         return 2;
     return 1;
@@ -483,22 +427,6 @@ extern "C" int gfd_operator_residual(gfd_operator* op, const void* in,
 
 // ================================================================ transfers
 
-struct gfd_transformer {
-    XferPlan plan{};
-    int periodic[3] = {0, 0, 0};
-    int real = 1;
-    DevBuf scratch;
-    long ngin() const
-    {
-        return (long)plan.nin[0] * plan.nin[1] * plan.nin[2];
-    }
-    long ngout() const
-    {
-        return (long)plan.nout[0] * plan.nout[1] * plan.nout[2];
-    }
-    int dtype() const { return real ? 0 : 1; }
-};
-
 extern "C" int gfd_transformer_new(const long size_in[3],
                                    const long size_out[3], int k,
                                    const long pad[6], const long npad[6],
@@ -585,7 +513,7 @@ static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
                    long nin, const double* phases, const XEpiArgs& ea,
                    hipStream_t s)
 {
-    const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);
+    const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);  // 3: c64
     if (t->plan.K != 2)
         GFD_TRY(t->scratch.ensure(es * xfer_scratch_elems(t->plan, nin) + 16,
                                   s));
@@ -620,13 +548,16 @@ static bool same3(const int a[3], const int b[3])
     return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
 }
 
-extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
-                              gfd_transformer* intp1, gfd_transformer* intp2,
-                              gfd_operator* kin0, gfd_operator* kin1,
-                              gfd_operator* kin2, void* d0, void* q0, void* r1,
-                              void* d1, void* q1, void* r2, void* d2, void* q2,
-                              const void* residuals, long nin, double step,
-                              const double* phases, void* stream)
+// fp32: the work arrays q0, r1, .., q2 are FP32 (f32 / c64) and every
+// pass computes in FP32; residuals and d0 stay FP64 (the first
+// restriction narrows, the last stencil pass widens).
+static int fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
+                      gfd_transformer* intp1, gfd_transformer* intp2,
+                      gfd_operator* kin0, gfd_operator* kin1,
+                      gfd_operator* kin2, void* d0, void* q0, void* r1,
+                      void* d1, void* q1, void* r2, void* d2, void* q2,
+                      const void* residuals, long nin, double step,
+                      const double* phases, void* stream, bool fp32)
 {
     if (!rest0 || !rest1 || !intp1 || !intp2 || !kin0 || !kin1 || !kin2 ||
         nin < 0) {
@@ -655,7 +586,7 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
     }
     hipStream_t s = (hipStream_t)stream;
     const double* ph = real ? nullptr : phases;
-    const int dt = real ? 0 : 1;
+    const int dt = fp32 ? (real ? 2 : 3) : (real ? 0 : 1);
     // r1 = R0(-res); d1 = 4 step r1
     {
         XEpiArgs e;
@@ -663,7 +594,8 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         e.negate = 1;
         e.alpha = 4 * step;
         e.out2 = d1;
-        GFD_TRY(xfer(rest0, residuals, r1, nin, ph, e, s));
+        e.in_wide = fp32 ? 1 : 0;
+        GFD_TRY(xfer_dt(rest0, dt, residuals, r1, nin, ph, e, s));
     }
     // q1 = K1 d1 - r1
     {
@@ -678,7 +610,7 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         e.epi = XEPI_SCALE2;
         e.alpha = 16 * step;
         e.out2 = d2;
-        GFD_TRY(xfer(rest1, q1, r2, nin, ph, e, s));
+        GFD_TRY(xfer_dt(rest1, dt, q1, r2, nin, ph, e, s));
     }
     // q2 = K2 d2 - r2; d2 -= 16 step q2        (new d2 held in q2)
     {
@@ -693,7 +625,7 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         XEpiArgs e;
         e.epi = XEPI_SUBFROM;
         e.base = d1;
-        GFD_TRY(xfer(intp2, q2, d1, nin, ph, e, s));
+        GFD_TRY(xfer_dt(intp2, dt, q2, d1, nin, ph, e, s));
     }
     // q1 = K1 d1 - r1; d1 -= 4 step q1          (new d1 held in q1)
     {
@@ -707,7 +639,7 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
     {
         XEpiArgs e;
         e.epi = XEPI_NEG;
-        GFD_TRY(xfer(intp1, q1, q0, nin, ph, e, s));
+        GFD_TRY(xfer_dt(intp1, dt, q1, q0, nin, ph, e, s));
     }
     // q0 = K0 d0' - res; d0 = -(d0' - step q0)
     {
@@ -716,9 +648,44 @@ extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         e.c0 = centre_coef(kin0);
         e.s = step;
         e.negate = 1;
-        GFD_TRY(op_pass(kin0, dt, q0, d0, nin, ph, EPI_UPDATE, e, s, nullptr));
+        GFD_TRY(op_pass(kin0, dt, q0, d0, nin, ph,
+                        fp32 ? EPI_UPDW : EPI_UPDATE, e, s, nullptr));
     }
     return OK;
+}
+
+extern "C" int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
+                              gfd_transformer* intp1, gfd_transformer* intp2,
+                              gfd_operator* kin0, gfd_operator* kin1,
+                              gfd_operator* kin2, void* d0, void* q0, void* r1,
+                              void* d1, void* q1, void* r2, void* d2, void* q2,
+                              const void* residuals, long nin, double step,
+                              const double* phases, void* stream)
+{
+    return fd_precond(rest0, rest1, intp1, intp2, kin0, kin1, kin2, d0, q0,
+                      r1, d1, q1, r2, d2, q2, residuals, nin, step, phases,
+                      stream, false);
+}
+
+extern "C" int gfd_fd_precond32(gfd_transformer* rest0,
+                                gfd_transformer* rest1,
+                                gfd_transformer* intp1,
+                                gfd_transformer* intp2, gfd_operator* kin0,
+                                gfd_operator* kin1, gfd_operator* kin2,
+                                void* d0, void* q0, void* r1, void* d1,
+                                void* q1, void* r2, void* d2, void* q2,
+                                const void* residuals, long nin, double step,
+                                const double* phases, void* stream)
+{
+    if (rest0 && rest1 && intp1 && intp2 &&
+        (rest0->plan.K != 2 || rest1->plan.K != 2 || intp1->plan.K != 2 ||
+         intp2->plan.K != 2)) {
+        set_error("gfd_fd_precond32: nn=1 transformers only");
+        return EINVAL_;
+    }
+    return fd_precond(rest0, rest1, intp1, intp2, kin0, kin1, kin2, d0, q0,
+                      r1, d1, q1, r2, d2, q2, residuals, nin, step, phases,
+                      stream, true);
 }
 
 // ================================================================= Poisson

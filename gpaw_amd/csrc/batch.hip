@@ -1,0 +1,1005 @@
+// Batched multigrid over many independent real grids of uneven shape.
+//
+// Every grid of a hierarchy level lives in one concatenated level array
+// (grid g at element offset off_g).  Each stencil stage of a level is ONE
+// persistent launch over the tiles of all grids: a block walks the tile
+// list with a grid stride, finds the tile's grid by binary search over the
+// per-grid first-tile table and applies that grid's axis stencil and
+// boundary (periodic by index wrap, zero by skipping).  Transfers are one
+// persistent launch per level pair with one thread per output point.  The
+// tile shape (threads along z and y, z points per thread) and the number
+// of resident blocks are picked per level by timing candidates on the
+// actual batch (gfd_batch_poisson_tune).
+//
+// The mixed-precision solve runs FP32 V-cycles (Jacobi smoothing, K = 2
+// transfers) on all grids together inside FP64 residual refinement; one
+// refinement step (FP64 residual + per-grid norms, FP32 V-cycle, FP64
+// update) is captured once as a hipGraph and replayed.
+#include "../../include/gpaw_fd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gfd_common.h"
+#include "handles.h"
+#include "stencil_dev.h"
+#include "vector.h"
+
+namespace gfd {
+
+constexpr int BMAXR = 3;
+
+// One grid of one level in a batched stencil launch.
+struct BSt {
+    long off;          // element offset in the level arrays
+    long tile0;        // first tile of the grid in the launch
+    int n[3];
+    int periodic[3];
+    int tz, ty;        // tiles along z and y (tiles = n0 * ty * tz)
+    int has[3];
+    double c0, w;
+    double cp[3][BMAXR], cm[3][BMAXR];
+};
+
+struct BSArgs {
+    const BSt* g;
+    int G;
+    long ntiles;
+    const void* in;
+    void* out;
+    const void* aux;
+    double* partial;   // one double per tile (EPI_RESID)
+};
+
+__device__ inline int bfind_tile(const BSt* g, int G, long t)
+{
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].tile0 <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// index along an axis with periodic wrap; false if outside a zero boundary
+__device__ inline bool bwrap(int& i, int n, int periodic)
+{
+    if (i >= 0 && i < n) return true;
+    if (!periodic) return false;
+    i %= n;
+    if (i < 0) i += n;
+    return true;
+}
+
+// T: data read, TO: data written (float output of an FP64 residual).
+// EPI_PLAIN, EPI_RELAX (aux = right-hand side) or EPI_RESID (aux = rhs,
+// partial[tile] = sum r^2).
+template <typename T, typename TO, int R, int PZ, int EPI>
+__global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
+{
+    typedef typename ScalarOf<T>::type S;
+    const int BZ = blockDim.x, BY = blockDim.y;
+    for (long t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const BSt& d = a.g[bfind_tile(a.g, a.G, t)];
+        long lt = t - d.tile0;
+        const int zt = (int)(lt % d.tz);
+        lt /= d.tz;
+        const int yt = (int)(lt % d.ty);
+        const int i0 = (int)(lt / d.ty);
+        const int i1 = yt * BY + threadIdx.y;
+        const int z0 = (zt * BZ + threadIdx.x) * PZ;
+        const int n0 = d.n[0], n1 = d.n[1], n2 = d.n[2];
+        double nrm = 0.0;
+        if (i1 < n1 && z0 < n2) {
+            const T* in = (const T*)a.in + d.off;
+            const long plane = (long)n1 * n2;
+            const T* row = in + (long)i0 * plane + (long)i1 * n2;
+            T acc[PZ];
+            T ctr[PZ];
+#pragma unroll
+            for (int p = 0; p < PZ; p++) acc[p] = zero_of<T>();
+            // z: a window of PZ + 2R points of the own row
+            {
+                T wz[PZ + 2 * R];
+#pragma unroll
+                for (int k = 0; k < PZ + 2 * R; k++) {
+                    int j = z0 - R + k;
+                    const bool in_row = j >= 0 && j < n2;
+                    T v = zero_of<T>();
+                    if (in_row || bwrap(j, n2, d.periodic[2])) v = row[j];
+                    wz[k] = v;
+                }
+#pragma unroll
+                for (int p = 0; p < PZ; p++) {
+                    ctr[p] = wz[R + p];
+                    if (d.has[2]) {
+#pragma unroll
+                        for (int k = 1; k <= R; k++) {
+                            acc[p] = fmac(acc[p], (S)d.cp[2][k - 1],
+                                          wz[R + p + k]);
+                            acc[p] = fmac(acc[p], (S)d.cm[2][k - 1],
+                                          wz[R + p - k]);
+                        }
+                    }
+                }
+            }
+            // y and x: PZ points of the neighbouring rows
+#pragma unroll
+            for (int ax = 1; ax >= 0; ax--) {
+                if (!d.has[ax]) continue;
+#pragma unroll
+                for (int k = 1; k <= R; k++) {
+#pragma unroll
+                    for (int sgn = 0; sgn < 2; sgn++) {
+                        int ii = ax == 1 ? i1 + (sgn ? -k : k)
+                                         : i0 + (sgn ? -k : k);
+                        if (!bwrap(ii, ax == 1 ? n1 : n0, d.periodic[ax]))
+                            continue;
+                        const T* nb = ax == 1
+                                          ? in + (long)i0 * plane +
+                                                (long)ii * n2
+                                          : in + (long)ii * plane +
+                                                (long)i1 * n2;
+                        const S c = sgn ? (S)d.cm[ax][k - 1]
+                                        : (S)d.cp[ax][k - 1];
+#pragma unroll
+                        for (int p = 0; p < PZ; p++)
+                            if (z0 + p < n2)
+                                acc[p] = fmac(acc[p], c, nb[z0 + p]);
+                    }
+                }
+            }
+            const long obase = d.off + (long)i0 * plane + (long)i1 * n2 + z0;
+            TO* out = (TO*)a.out;
+#pragma unroll
+            for (int p = 0; p < PZ; p++) {
+                if (z0 + p >= n2) break;
+                T r;
+                if constexpr (EPI == EPI_PLAIN) {
+                    r = fmac(acc[p], (S)d.c0, ctr[p]);
+                } else if constexpr (EPI == EPI_RELAX) {
+                    const T f = ((const T*)a.aux)[obase + p];
+                    r = ((S)1.0 - (S)d.w) * ctr[p] +
+                        (S)d.w * (f - acc[p]) / (S)d.c0;
+                } else {
+                    const T f = ((const T*)a.aux)[obase + p];
+                    r = fmac(acc[p], (S)d.c0, ctr[p]) - f;
+                    nrm += (double)r * (double)r;
+                }
+                out[obase + p] = (TO)r;
+            }
+        }
+        if (EPI == EPI_RESID && a.partial) block_sum_store(nrm, a.partial, t);
+    }
+}
+
+// per-grid sums of the tile partials: one block per grid
+__global__ void __launch_bounds__(256)
+k_bnorms(const double* __restrict__ partial, const BSt* __restrict__ g, int G,
+         long ntiles, double* __restrict__ out)
+{
+    const int gi = blockIdx.x;
+    const long t0 = g[gi].tile0;
+    const long t1 = gi + 1 < G ? g[gi + 1].tile0 : ntiles;
+    double s = 0.0;
+    for (long t = t0 + threadIdx.x; t < t1; t += 256) s += partial[t];
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[gi] = red[0];
+}
+
+// One grid pair of a batched K = 2 transfer.
+struct BXf {
+    long in_off, out_off;
+    long p0;           // first output point of the grid in the launch
+    int ni[3], no[3];
+    int off[3], skip0[3];
+    int periodic[3];
+};
+
+struct BXArgs {
+    const BXf* g;
+    int G;
+    long total;
+    const void* in;
+    void* out;
+    const void* base;  // interpolation: out = base - I(in)
+};
+
+__device__ inline int bfind_point(const BXf* g, int G, long p)
+{
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].p0 <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ inline T bxload(const BXf& d, const T* __restrict__ g, int i0,
+                           int i1, int i2)
+{
+    if (!bwrap(i0, d.ni[0], d.periodic[0]) ||
+        !bwrap(i1, d.ni[1], d.periodic[1]) ||
+        !bwrap(i2, d.ni[2], d.periodic[2]))
+        return zero_of<T>();
+    return g[((long)i0 * d.ni[1] + i1) * d.ni[2] + i2];
+}
+
+// Restriction (INTERP false) or interpolation with out = base - I(in)
+// (INTERP true), K = 2 weights, the association order of the single-grid
+// fused kernels (z innermost, then y, then x).
+template <typename T, bool INTERP>
+__global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
+{
+    typedef typename ScalarOf<T>::type S;
+    for (long gid = (long)blockIdx.x * 256 + threadIdx.x; gid < a.total;
+         gid += (long)gridDim.x * 256) {
+        const BXf& d = a.g[bfind_point(a.g, a.G, gid)];
+        long r = gid - d.p0;
+        const int o2 = (int)(r % d.no[2]);
+        r /= d.no[2];
+        const int o1 = (int)(r % d.no[1]);
+        const int o0 = (int)(r / d.no[1]);
+        const T* __restrict__ g = (const T*)a.in + d.in_off;
+        const long oidx =
+            d.out_off + ((long)o0 * d.no[1] + o1) * d.no[2] + o2;
+        T v;
+        if constexpr (!INTERP) {
+            const int c0 = 2 * o0 + d.off[0], c1 = 2 * o1 + d.off[1],
+                      c2 = 2 * o2 + d.off[2];
+            T px[3];
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                T py[3];
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++) {
+                    const T zp = bxload(d, g, c0 + dx, c1 + dy, c2 + 1) +
+                                 bxload(d, g, c0 + dx, c1 + dy, c2 - 1);
+                    py[dy + 1] = (S)0.5 * fmac(bxload(d, g, c0 + dx, c1 + dy,
+                                                      c2),
+                                               (S)0.5, zp);
+                }
+                px[dx + 1] = (S)0.5 * fmac(py[1], (S)0.5, py[2] + py[0]);
+            }
+            v = (S)0.5 * fmac(px[1], (S)0.5, px[2] + px[0]);
+            ((T*)a.out)[oidx] = v;
+        } else {
+            const int q0 = o0 + d.skip0[0], q1 = o1 + d.skip0[1],
+                      q2 = o2 + d.skip0[2];
+            const int c0 = (q0 >> 1) + d.off[0], c1 = (q1 >> 1) + d.off[1],
+                      c2 = (q2 >> 1) + d.off[2];
+            const int m0 = (q0 & 1) + 1, m1 = (q1 & 1) + 1, m2 = (q2 & 1) + 1;
+            T vx[2];
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++) {
+                if (dx >= m0) break;
+                T vy[2];
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    if (dy >= m1) break;
+                    T vz = bxload(d, g, c0 + dx, c1 + dy, c2);
+                    if (m2 == 2)
+                        vz = (S)0.5 *
+                             (vz + bxload(d, g, c0 + dx, c1 + dy, c2 + 1));
+                    vy[dy] = vz;
+                }
+                vx[dx] = (m1 == 2) ? (S)0.5 * (vy[0] + vy[1]) : vy[0];
+            }
+            v = (m0 == 2) ? (S)0.5 * (vx[0] + vx[1]) : vx[0];
+            ((T*)a.out)[oidx] = ((const T*)a.base)[oidx] - v;
+        }
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+// Tile shape of one level: z threads, y threads, z points per thread,
+// resident blocks (0: one block per tile).
+struct BTile {
+    int bz = 64, by = 4, pz = 1;
+    long blocks = 0;
+};
+
+struct BLevel {
+    int R = 1;
+    long total = 0;                 // points of all grids
+    std::vector<BSt> st;            // per grid (tile0 for `tile`)
+    BTile tile;
+    long ntiles = 0;
+    DevBuf d_st;
+};
+
+struct BPair {                      // transfer from level l to l + 1
+    std::vector<BXf> rst, itp;
+    long nr = 0, ni = 0;            // output points
+    DevBuf d_rst, d_itp;
+};
+
+}  // namespace gfd
+
+using namespace gfd;
+
+struct gfd_batch_poisson {
+    int G = 0, L = 0;
+    std::vector<BLevel> lev;
+    std::vector<BPair> pair;
+    BLevel l0_64;                   // level 0 for the FP64 residual
+    bool tuned = false;
+    // FP64 level-0 arrays, FP32 arrays per level
+    DevBuf phi, rho;
+    std::vector<DevBuf> e32, t32, r32, s32;
+    DevBuf partial, norms;
+    std::vector<double> rn2;
+    hipStream_t gs = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int gkey[4] = {-1, -1, -1, -1};
+    int num_cu = 256;
+};
+
+namespace gfd {
+
+static void fill_tiles(BLevel& lv, const BTile& t)
+{
+    lv.tile = t;
+    long t0 = 0;
+    for (BSt& s : lv.st) {
+        s.tz = (s.n[2] + t.bz * t.pz - 1) / (t.bz * t.pz);
+        s.ty = (s.n[1] + t.by - 1) / t.by;
+        s.tile0 = t0;
+        t0 += (long)s.n[0] * s.ty * s.tz;
+    }
+    lv.ntiles = t0;
+}
+
+static int upload_level(BLevel& lv, hipStream_t s)
+{
+    GFD_TRY(lv.d_st.ensure(sizeof(BSt) * lv.st.size(), s));
+    GFD_CHECK_HIP(hipMemcpyAsync(lv.d_st.p, lv.st.data(),
+                                 sizeof(BSt) * lv.st.size(),
+                                 hipMemcpyHostToDevice, s));
+    return hipStreamSynchronize(s) == hipSuccess ? OK : EHIP;
+}
+
+template <typename T, typename TO, int R, int PZ, int EPI>
+static int bst_launch_t(const BLevel& lv, const void* in, void* out,
+                        const void* aux, double* partial, hipStream_t s)
+{
+    BSArgs a;
+    a.g = (const BSt*)lv.d_st.p;
+    a.G = (int)lv.st.size();
+    a.ntiles = lv.ntiles;
+    a.in = in;
+    a.out = out;
+    a.aux = aux;
+    a.partial = partial;
+    long nb = lv.tile.blocks > 0 ? std::min(lv.tile.blocks, lv.ntiles)
+                                 : lv.ntiles;
+    if (nb <= 0) return OK;
+    if (nb > 0x7fffffffL) nb = 0x7fffffffL;
+    hipLaunchKernelGGL((k_bstencil<T, TO, R, PZ, EPI>), dim3((unsigned)nb),
+                       dim3(lv.tile.bz, lv.tile.by), 0, s, a);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+template <typename T, typename TO, int EPI>
+static int bst_launch_r(const BLevel& lv, const void* in, void* out,
+                        const void* aux, double* partial, hipStream_t s)
+{
+#define GFD_BST(RR, PP)                                                       \
+    if (lv.R == RR && lv.tile.pz == PP)                                       \
+        return bst_launch_t<T, TO, RR, PP, EPI>(lv, in, out, aux, partial, s);
+    GFD_BST(1, 1) GFD_BST(1, 2) GFD_BST(1, 4)
+    GFD_BST(2, 1) GFD_BST(2, 2) GFD_BST(2, 4)
+    GFD_BST(3, 1) GFD_BST(3, 2) GFD_BST(3, 4)
+#undef GFD_BST
+    set_error("batched stencil: radius %d / %d z points not built", lv.R,
+              lv.tile.pz);
+    return ENOTSUP_;
+}
+
+// dtype 0: f64 -> f64, 2: f32 -> f32, 4: f64 -> f32 (RESID only)
+static int bst(const BLevel& lv, int dtype, int epi, const void* in,
+               void* out, const void* aux, double* partial, hipStream_t s)
+{
+    if (dtype == 0 && epi == EPI_PLAIN)
+        return bst_launch_r<double, double, EPI_PLAIN>(lv, in, out, aux,
+                                                       partial, s);
+    if (dtype == 2 && epi == EPI_PLAIN)
+        return bst_launch_r<float, float, EPI_PLAIN>(lv, in, out, aux,
+                                                     partial, s);
+    if (dtype == 2 && epi == EPI_RELAX)
+        return bst_launch_r<float, float, EPI_RELAX>(lv, in, out, aux,
+                                                     partial, s);
+    if (dtype == 2 && epi == EPI_RESID)
+        return bst_launch_r<float, float, EPI_RESID>(lv, in, out, aux,
+                                                     partial, s);
+    if (dtype == 4 && epi == EPI_RESID)
+        return bst_launch_r<double, float, EPI_RESID>(lv, in, out, aux,
+                                                      partial, s);
+    set_error("batched stencil: dtype %d / epilogue %d not built", dtype,
+              epi);
+    return ENOTSUP_;
+}
+
+static int bxfer(const BPair& pr, bool interp, int dtype, const void* in,
+                 void* out, const void* base, int num_cu, hipStream_t s)
+{
+    BXArgs a;
+    a.g = (const BXf*)(interp ? pr.d_itp.p : pr.d_rst.p);
+    a.G = (int)(interp ? pr.itp.size() : pr.rst.size());
+    a.total = interp ? pr.ni : pr.nr;
+    a.in = in;
+    a.out = out;
+    a.base = base;
+    if (a.total <= 0) return OK;
+    const long nb = std::min((a.total + 255) / 256, (long)num_cu * 16);
+    if (dtype == 0) {
+        if (interp)
+            hipLaunchKernelGGL((k_bxfer2<double, true>), dim3((unsigned)nb),
+                               dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_bxfer2<double, false>), dim3((unsigned)nb),
+                               dim3(256), 0, s, a);
+    } else {
+        if (interp)
+            hipLaunchKernelGGL((k_bxfer2<float, true>), dim3((unsigned)nb),
+                               dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_bxfer2<float, false>), dim3((unsigned)nb),
+                               dim3(256), 0, s, a);
+    }
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+static int bnorms(const BLevel& lv, const double* partial, double* out,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bnorms, dim3((unsigned)lv.st.size()), dim3(256), 0,
+                       s, partial, (const BSt*)lv.d_st.p, (int)lv.st.size(),
+                       lv.ntiles, out);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+// Candidate tile shapes, timed with a PLAIN pass over the level.
+static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
+                      const void* in, void* out, hipStream_t s)
+{
+    static const int shapes[][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16},
+                                    {64, 2}, {32, 4}};
+    static const int pzs[] = {1, 2, 4};
+    const long res[] = {0, (long)b->num_cu * 4, (long)b->num_cu * 8};
+    hipEvent_t e0, e1;
+    GFD_CHECK_HIP(hipEventCreate(&e0));
+    GFD_CHECK_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    BTile bt;
+    int st = OK;
+    for (const auto& sh : shapes) {
+        for (int pz : pzs) {
+            for (long nb : res) {
+                BTile t;
+                t.bz = sh[0];
+                t.by = sh[1];
+                t.pz = pz;
+                t.blocks = nb;
+                fill_tiles(lv, t);
+                if ((st = upload_level(lv, s)) != OK) break;
+                const int ep = EPI_PLAIN;
+                const int dt = dtype == 4 ? 0 : dtype;
+                if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) != OK)
+                    break;
+                (void)hipEventRecord(e0, s);
+                for (int r = 0; r < 3; r++)
+                    if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) !=
+                        OK)
+                        break;
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (st == OK && ms < best) {
+                    best = ms;
+                    bt = t;
+                }
+            }
+            if (st != OK) break;
+        }
+        if (st != OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != OK) return st;
+    fill_tiles(lv, bt);
+    return upload_level(lv, s);
+}
+
+}  // namespace gfd
+
+extern "C" int gfd_batch_poisson_new(int G, int L, gfd_operator** ops,
+                                     gfd_transformer** rst,
+                                     gfd_transformer** itp,
+                                     const double* weights,
+                                     gfd_batch_poisson** out)
+{
+    if (!out || G < 1 || L < 1 || !ops || !weights ||
+        (L > 1 && (!rst || !itp))) {
+        set_error("gfd_batch_poisson_new: bad arguments");
+        return EINVAL_;
+    }
+    *out = nullptr;
+    auto* b = new gfd_batch_poisson;
+    b->G = G;
+    b->L = L;
+    b->lev.resize(L);
+    b->pair.resize(L > 1 ? L - 1 : 0);
+    for (int l = 0; l < L; l++) {
+        BLevel& lv = b->lev[l];
+        long off = 0;
+        for (int g = 0; g < G; g++) {
+            gfd_operator* op = ops[g * L + l];
+            if (!op || !op->real || !op->axis_ok || op->ax.R > BMAXR) {
+                set_error("batched Poisson: grid %d level %d needs a real "
+                          "axis-only stencil of radius <= %d",
+                          g, l, BMAXR);
+                delete b;
+                return EINVAL_;
+            }
+            if (g == 0) lv.R = op->ax.R;
+            if (op->ax.R != lv.R) {
+                set_error("batched Poisson: level %d mixes stencil radii", l);
+                delete b;
+                return EINVAL_;
+            }
+            BSt s{};
+            s.off = off;
+            for (int c = 0; c < 3; c++) {
+                s.n[c] = op->n[c];
+                s.periodic[c] = op->periodic[c];
+                s.has[c] = op->ax.has[c];
+                for (int k = 0; k < BMAXR; k++) {
+                    s.cp[c][k] = k < op->ax.R ? op->ax.cp[c][k] : 0.0;
+                    s.cm[c][k] = k < op->ax.R ? op->ax.cm[c][k] : 0.0;
+                }
+                if (op->n[c] < op->ax.R) {
+                    set_error("batched Poisson: grid %d level %d smaller "
+                              "than the stencil",
+                              g, l);
+                    delete b;
+                    return EINVAL_;
+                }
+            }
+            s.c0 = op->ax.c0;
+            s.w = weights[l];
+            lv.st.push_back(s);
+            off += op->ng();
+        }
+        lv.total = off;
+        fill_tiles(lv, BTile());
+    }
+    b->l0_64 = BLevel();
+    b->l0_64.R = b->lev[0].R;
+    b->l0_64.st = b->lev[0].st;
+    b->l0_64.total = b->lev[0].total;
+    fill_tiles(b->l0_64, BTile());
+    for (int l = 0; l + 1 < L; l++) {
+        BPair& pr = b->pair[l];
+        long ioff = 0, ooff = 0, p0r = 0, p0i = 0;
+        for (int g = 0; g < G; g++) {
+            gfd_transformer* r = rst[g * (L - 1) + l];
+            gfd_transformer* i = itp[g * (L - 1) + l];
+            if (!r || !i || !r->real || !i->real || r->plan.K != 2 ||
+                i->plan.K != 2 || r->plan.interpolate ||
+                !i->plan.interpolate) {
+                set_error("batched Poisson: grid %d level %d needs real nn=1 "
+                          "restriction / interpolation",
+                          g, l);
+                delete b;
+                return EINVAL_;
+            }
+            const BSt& f = b->lev[l].st[g];
+            const BSt& c = b->lev[l + 1].st[g];
+            for (int d = 0; d < 3; d++)
+                if (r->plan.nin[d] != f.n[d] || r->plan.nout[d] != c.n[d] ||
+                    i->plan.nin[d] != c.n[d] || i->plan.nout[d] != f.n[d]) {
+                    set_error("batched Poisson: grid %d level %d transfer "
+                              "shapes do not match the operators",
+                              g, l);
+                    delete b;
+                    return EINVAL_;
+                }
+            BXf x{};
+            x.in_off = f.off;
+            x.out_off = c.off;
+            x.p0 = p0r;
+            for (int d = 0; d < 3; d++) {
+                x.ni[d] = f.n[d];
+                x.no[d] = c.n[d];
+                x.off[d] = r->plan.ax[d].off;
+                x.skip0[d] = r->plan.ax[d].skip0;
+                x.periodic[d] = r->periodic[d];
+            }
+            pr.rst.push_back(x);
+            p0r += c.n[0] * (long)c.n[1] * c.n[2];
+            BXf y{};
+            y.in_off = c.off;
+            y.out_off = f.off;
+            y.p0 = p0i;
+            for (int d = 0; d < 3; d++) {
+                y.ni[d] = c.n[d];
+                y.no[d] = f.n[d];
+                y.off[d] = i->plan.ax[d].off;
+                y.skip0[d] = i->plan.ax[d].skip0;
+                y.periodic[d] = i->periodic[d];
+            }
+            pr.itp.push_back(y);
+            p0i += f.n[0] * (long)f.n[1] * f.n[2];
+            (void)ioff;
+            (void)ooff;
+        }
+        pr.nr = p0r;
+        pr.ni = p0i;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) == hipSuccess)
+            b->num_cu = pr.multiProcessorCount;
+    }
+    (void)hipGetLastError();
+    b->e32.resize(L);
+    b->t32.resize(L);
+    b->r32.resize(L);
+    b->s32.resize(L);
+    *out = b;
+    return OK;
+}
+
+extern "C" void gfd_batch_poisson_free(gfd_batch_poisson* b)
+{
+    if (!b) return;
+    if (b->gs) (void)hipStreamSynchronize(b->gs);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto& lv : b->lev) lv.d_st.release();
+    b->l0_64.d_st.release();
+    for (auto& pr : b->pair) {
+        pr.d_rst.release();
+        pr.d_itp.release();
+    }
+    for (int l = 0; l < b->L; l++) {
+        b->e32[l].release();
+        b->t32[l].release();
+        b->r32[l].release();
+        b->s32[l].release();
+    }
+    b->phi.release();
+    b->rho.release();
+    b->partial.release();
+    b->norms.release();
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    if (b->gs) (void)hipStreamDestroy(b->gs);
+    delete b;
+}
+
+extern "C" long gfd_batch_poisson_level_points(gfd_batch_poisson* b,
+                                               int level)
+{
+    if (!b || level < 0 || level >= b->L) return -1;
+    return b->lev[level].total;
+}
+
+namespace gfd {
+
+static int b_alloc(gfd_batch_poisson* b, hipStream_t s)
+{
+    GFD_TRY(b->phi.ensure(8 * b->lev[0].total, s));
+    GFD_TRY(b->rho.ensure(8 * b->lev[0].total, s));
+    for (int l = 0; l < b->L; l++) {
+        const size_t n = 4 * (size_t)b->lev[l].total + 16;
+        GFD_TRY(b->e32[l].ensure(n, s));
+        GFD_TRY(b->t32[l].ensure(n, s));
+        GFD_TRY(b->r32[l].ensure(n, s));
+        GFD_TRY(b->s32[l].ensure(n, s));
+    }
+    for (auto& pr : b->pair) {
+        GFD_TRY(pr.d_rst.ensure(sizeof(BXf) * pr.rst.size(), s));
+        GFD_TRY(pr.d_itp.ensure(sizeof(BXf) * pr.itp.size(), s));
+        GFD_CHECK_HIP(hipMemcpyAsync(pr.d_rst.p, pr.rst.data(),
+                                     sizeof(BXf) * pr.rst.size(),
+                                     hipMemcpyHostToDevice, s));
+        GFD_CHECK_HIP(hipMemcpyAsync(pr.d_itp.p, pr.itp.data(),
+                                     sizeof(BXf) * pr.itp.size(),
+                                     hipMemcpyHostToDevice, s));
+    }
+    GFD_TRY(b->norms.ensure(sizeof(double) * b->G, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+static int b_tune(gfd_batch_poisson* b, hipStream_t s)
+{
+    GFD_TRY(b_alloc(b, s));
+    for (int l = 0; l < b->L; l++) {
+        BLevel& lv = b->lev[l];
+        GFD_TRY(vec_zero_bytes(b->e32[l].p, 4 * lv.total, s));
+        GFD_TRY(tune_level(b, lv, 2, b->e32[l].p, b->t32[l].p, s));
+    }
+    GFD_TRY(vec_zero(( double*)b->phi.p, b->lev[0].total, s));
+    GFD_TRY(tune_level(b, b->l0_64, 0, b->phi.p, b->rho.p, s));
+    long mt = 0;
+    for (auto& lv : b->lev) mt = std::max(mt, lv.ntiles);
+    mt = std::max(mt, b->l0_64.ntiles);
+    GFD_TRY(b->partial.ensure(sizeof(double) * (mt + 1), s));
+    b->tuned = true;
+    return OK;
+}
+
+// Jacobi sweeps at level l (FP32), result in *cur
+static int b_relax(gfd_batch_poisson* b, int l, void** cur, void** alt,
+                   const void* rhs, int n, hipStream_t s)
+{
+    for (int i = 0; i < n; i++) {
+        GFD_TRY(bst(b->lev[l], 2, EPI_RELAX, *cur, *alt, rhs, nullptr, s));
+        std::swap(*cur, *alt);
+    }
+    return OK;
+}
+
+// FP32 V-cycle on e_l (zero on entry) for right-hand side r_l
+static int b_vcycle(gfd_batch_poisson* b, int l, void** cur, void** alt,
+                    const void* rhs, int pre, int post, int coarse,
+                    hipStream_t s)
+{
+    if (l + 1 < b->L) {
+        GFD_TRY(b_relax(b, l, cur, alt, rhs, pre, s));
+        GFD_TRY(bst(b->lev[l], 2, EPI_RESID, *cur, b->s32[l].p, rhs, nullptr,
+                    s));
+        GFD_TRY(bxfer(b->pair[l], false, 2, b->s32[l].p, b->r32[l + 1].p,
+                      nullptr, b->num_cu, s));
+        void* c1 = b->e32[l + 1].p;
+        void* a1 = b->t32[l + 1].p;
+        GFD_TRY(vec_zero_bytes(c1, 4 * b->lev[l + 1].total, s));
+        GFD_TRY(b_vcycle(b, l + 1, &c1, &a1, b->r32[l + 1].p, pre, post,
+                         coarse, s));
+        GFD_TRY(bxfer(b->pair[l], true, 2, c1, *alt, *cur, b->num_cu, s));
+        std::swap(*cur, *alt);
+        return b_relax(b, l, cur, alt, rhs, post, s);
+    }
+    return b_relax(b, l, cur, alt, rhs, coarse, s);
+}
+
+// One refinement step: e = V(r32), phi -= e, r = L phi - rho (FP64, per-grid
+// norms), r32 = (float) r.  Entry: r32[0] holds the current residual.
+static int b_step(gfd_batch_poisson* b, int pre, int post, int coarse,
+                  hipStream_t s)
+{
+    const long n0 = b->lev[0].total;
+    void* cur = b->e32[0].p;
+    void* alt = b->t32[0].p;
+    GFD_TRY(vec_zero_bytes(cur, 4 * n0, s));
+    GFD_TRY(b_vcycle(b, 0, &cur, &alt, b->r32[0].p, pre, post, coarse, s));
+    GFD_TRY(vec_sub_f((double*)b->phi.p, (const float*)cur, n0, s));
+    GFD_TRY(bst(b->l0_64, 4, EPI_RESID, b->phi.p, b->r32[0].p, b->rho.p,
+                (double*)b->partial.p, s));
+    return bnorms(b->l0_64, (const double*)b->partial.p,
+                  (double*)b->norms.p, s);
+}
+
+static int b_read_norms(gfd_batch_poisson* b, std::vector<double>& h,
+                        hipStream_t s)
+{
+    h.resize(b->G);
+    GFD_CHECK_HIP(hipMemcpyAsync(h.data(), b->norms.p, sizeof(double) * b->G,
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+}  // namespace gfd
+
+extern "C" int gfd_batch_poisson_tune(gfd_batch_poisson* b, void* stream)
+{
+    if (!b) {
+        set_error("gfd_batch_poisson_tune: null handle");
+        return EINVAL_;
+    }
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return b_tune(b, (hipStream_t)stream);
+}
+
+extern "C" int gfd_batch_poisson_tiles(gfd_batch_poisson* b, int level,
+                                       int* bz, int* by, int* pz,
+                                       long* blocks)
+{
+    if (!b || level < -1 || level >= b->L || !bz || !by || !pz || !blocks) {
+        set_error("gfd_batch_poisson_tiles: bad arguments");
+        return EINVAL_;
+    }
+    const BTile& t = level < 0 ? b->l0_64.tile : b->lev[level].tile;
+    *bz = t.bz;
+    *by = t.by;
+    *pz = t.pz;
+    *blocks = t.blocks;
+    return OK;
+}
+
+extern "C" int gfd_batch_poisson_apply(gfd_batch_poisson* b, int level,
+                                       int fp32, const void* in, void* out,
+                                       void* stream)
+{
+    if (!b || level < 0 || level >= b->L || !in || !out || in == out) {
+        set_error("gfd_batch_poisson_apply: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->tuned) GFD_TRY(b_tune(b, s));
+    return bst(b->lev[level], fp32 ? 2 : 0, EPI_PLAIN, in, out, nullptr,
+               nullptr, s);
+}
+
+extern "C" int gfd_batch_poisson_transfer(gfd_batch_poisson* b, int level,
+                                          int interpolate, int fp32,
+                                          const void* in, void* out,
+                                          const void* base, void* stream)
+{
+    if (!b || level < 0 || level + 1 >= b->L || !in || !out ||
+        (interpolate && !base)) {
+        set_error("gfd_batch_poisson_transfer: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->tuned) GFD_TRY(b_tune(b, s));
+    return bxfer(b->pair[level], interpolate != 0, fp32 ? 2 : 0, in, out,
+                 base, b->num_cu, s);
+}
+
+extern "C" int gfd_batch_poisson_solve_mixed(
+    gfd_batch_poisson* b, double* const* phis, const double* const* rhos,
+    double tol_rel, int pre, int post, int coarse_sweeps, int max_outer,
+    int use_graph, int* n_outer, double* relres, void* stream)
+{
+    if (!b || !phis || !rhos || !n_outer || !relres || max_outer < 1 ||
+        pre < 0 || post < 0 || coarse_sweeps < 0) {
+        set_error("gfd_batch_poisson_solve_mixed: bad arguments");
+        return EINVAL_;
+    }
+    for (int g = 0; g < b->G; g++)
+        if (!phis[g] || !rhos[g]) {
+            set_error("gfd_batch_poisson_solve_mixed: null grid %d", g);
+            return EINVAL_;
+        }
+    hipStream_t cs = (hipStream_t)stream;
+    if (!b->gs) {
+        GFD_CHECK_HIP(hipStreamCreateWithFlags(&b->gs, hipStreamNonBlocking));
+        GFD_CHECK_HIP(hipEventCreateWithFlags(&b->ev_in,
+                                              hipEventDisableTiming));
+        GFD_CHECK_HIP(hipEventCreateWithFlags(&b->ev_out,
+                                              hipEventDisableTiming));
+    }
+    hipStream_t s = b->gs;
+    GFD_CHECK_HIP(hipEventRecord(b->ev_in, cs));
+    GFD_CHECK_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
+    if (!b->tuned) GFD_TRY(b_tune(b, s));
+    // gather phi, rho; |rho_g|^2
+    const BLevel& l0 = b->lev[0];
+    std::vector<double> rn2(b->G);
+    GFD_TRY(b->norms.ensure(sizeof(double) * b->G, s));
+    for (int g = 0; g < b->G; g++) {
+        const long ng = (long)l0.st[g].n[0] * l0.st[g].n[1] * l0.st[g].n[2];
+        double* pd = (double*)b->phi.p + l0.st[g].off;
+        double* rd = (double*)b->rho.p + l0.st[g].off;
+        GFD_CHECK_HIP(hipMemcpyAsync(pd, phis[g], 8 * ng,
+                                     hipMemcpyDeviceToDevice, s));
+        GFD_CHECK_HIP(hipMemcpyAsync(rd, rhos[g], 8 * ng,
+                                     hipMemcpyDeviceToDevice, s));
+    }
+    // r = L phi - rho with norms; rn2 from the same pass on phi = 0 is not
+    // needed: |rho|^2 by a dot per grid
+    {
+        DevBuf tmp;
+        const long ng0 = l0.total;
+        GFD_TRY(tmp.ensure(sizeof(double) * (vec_dot_partials(ng0) + 8), s));
+        for (int g = 0; g < b->G; g++) {
+            const long ng =
+                (long)l0.st[g].n[0] * l0.st[g].n[1] * l0.st[g].n[2];
+            const double* rd = (const double*)b->rho.p + l0.st[g].off;
+            GFD_TRY(vec_dot(rd, rd, ng, (double*)tmp.p + 1, (double*)tmp.p,
+                            s));
+            GFD_CHECK_HIP(hipMemcpyAsync(&rn2[g], tmp.p, sizeof(double),
+                                         hipMemcpyDeviceToHost, s));
+            GFD_CHECK_HIP(hipStreamSynchronize(s));
+        }
+        tmp.release();
+    }
+    GFD_TRY(bst(b->l0_64, 4, EPI_RESID, b->phi.p, b->r32[0].p, b->rho.p,
+                (double*)b->partial.p, s));
+    GFD_TRY(bnorms(b->l0_64, (const double*)b->partial.p,
+                   (double*)b->norms.p, s));
+    std::vector<double> err2;
+    GFD_TRY(b_read_norms(b, err2, s));
+    auto done = [&]() {
+        for (int g = 0; g < b->G; g++)
+            if (err2[g] > tol_rel * tol_rel * rn2[g]) return false;
+        return true;
+    };
+    const int key[4] = {pre, post, coarse_sweeps, 0};
+    if (b->graph && !std::equal(key, key + 4, b->gkey)) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    int outer = 0;
+    while (!done() && outer < max_outer) {
+        if (use_graph) {
+            if (!b->graph) {
+                hipGraph_t gr = nullptr;
+                GFD_CHECK_HIP(hipStreamBeginCapture(
+                    s, hipStreamCaptureModeThreadLocal));
+                const int st = b_step(b, pre, post, coarse_sweeps, s);
+                const hipError_t e = hipStreamEndCapture(s, &gr);
+                if (st != OK) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return st;
+                }
+                if (e != hipSuccess) {
+                    set_error("graph capture failed: %s",
+                              hipGetErrorString(e));
+                    return EHIP;
+                }
+                const hipError_t e2 =
+                    hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e2 != hipSuccess) {
+                    b->graph = nullptr;
+                    set_error("graph instantiation failed: %s",
+                              hipGetErrorString(e2));
+                    return EHIP;
+                }
+                std::copy(key, key + 4, b->gkey);
+            }
+            GFD_CHECK_HIP(hipGraphLaunch(b->graph, s));
+        } else {
+            GFD_TRY(b_step(b, pre, post, coarse_sweeps, s));
+        }
+        GFD_TRY(b_read_norms(b, err2, s));
+        outer++;
+    }
+    for (int g = 0; g < b->G; g++) {
+        const long ng = (long)l0.st[g].n[0] * l0.st[g].n[1] * l0.st[g].n[2];
+        GFD_CHECK_HIP(hipMemcpyAsync(phis[g],
+                                     (double*)b->phi.p + l0.st[g].off, 8 * ng,
+                                     hipMemcpyDeviceToDevice, s));
+        relres[g] = rn2[g] > 0 ? std::sqrt(err2[g] / rn2[g])
+                               : std::sqrt(err2[g]);
+    }
+    GFD_CHECK_HIP(hipEventRecord(b->ev_out, s));
+    GFD_CHECK_HIP(hipStreamWaitEvent(cs, b->ev_out, 0));
+    *n_outer = outer;
+    if (!done()) {
+        set_error("batched mixed-precision Poisson: not converged to %g in "
+                  "%d refinement steps",
+                  tol_rel, max_outer);
+        return ECONVERGE;
+    }
+    return OK;
+}

@@ -80,12 +80,35 @@ __host__ __device__ inline double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+__host__ __device__ inline float2 operator+(float2 a, float2 b)
+{
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__host__ __device__ inline float2 operator-(float2 a, float2 b)
+{
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__host__ __device__ inline float2 operator*(float s, float2 a)
+{
+    return make_float2(s * a.x, s * a.y);
+}
+__host__ __device__ inline float2 operator-(float2 a)
+{
+    return make_float2(-a.x, -a.y);
+}
+__host__ __device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // Scalar-generic helpers used by the templated kernels (T = double|double2
-// for the FP64 paths, float|float2 for the mixed-precision smoothers).
+// for the FP64 paths, float|float2 for the mixed-precision smoothers and
+// preconditioner).
 template <typename T> struct ScalarOf;
 template <> struct ScalarOf<double> { typedef double type; };
 template <> struct ScalarOf<double2> { typedef double type; };
 template <> struct ScalarOf<float> { typedef float type; };
+template <> struct ScalarOf<float2> { typedef float type; };
 
 template <typename T>
 __device__ inline T zero_of();
@@ -94,6 +117,10 @@ template <> __device__ inline float zero_of<float>() { return 0.0f; }
 template <> __device__ inline double2 zero_of<double2>()
 {
     return make_double2(0.0, 0.0);
+}
+template <> __device__ inline float2 zero_of<float2>()
+{
+    return make_float2(0.0f, 0.0f);
 }
 
 // acc + c * v with c real
@@ -109,6 +136,10 @@ __device__ inline double2 fmac(double2 acc, double c, double2 v)
 {
     return make_double2(fma(c, v.x, acc.x), fma(c, v.y, acc.y));
 }
+__device__ inline float2 fmac(float2 acc, float c, float2 v)
+{
+    return make_float2(fmaf(c, v.x, acc.x), fmaf(c, v.y, acc.y));
+}
 
 // value times a boundary factor (1, 0 or a Bloch phase)
 __device__ inline double apply_phase(double v, double2 ph) { return ph.x * v; }
@@ -119,6 +150,10 @@ __device__ inline float apply_phase(float v, double2 ph)
 __device__ inline double2 apply_phase(double2 v, double2 ph)
 {
     return cmul(ph, v);
+}
+__device__ inline float2 apply_phase(float2 v, double2 ph)
+{
+    return cmul(make_float2((float)ph.x, (float)ph.y), v);
 }
 
 // ---------------------------------------------------------------- grids

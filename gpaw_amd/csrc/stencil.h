@@ -17,6 +17,7 @@ enum Epi : int {
     EPI_RESID = 2,   // out = (c0 v + acc) - aux          [+ sum out^2]
     EPI_UPDATE = 3,  // out = +-(v - s ((c0 v + acc) - aux))
     EPI_POT = 4,     // out = (c0 v + acc) + pot v, pot real per grid point
+    EPI_UPDW = 5,    // EPI_UPDATE on FP32 data with aux and out in FP64
                      // (shared by the grids of a batch): H psi
 };
 
@@ -56,7 +57,8 @@ struct XmTile {
     int xchunk;  // planes per block along x
 };
 // Launch one stencil pass over `nin` grids of shape n[3].
-// dtype: 0 = f64, 1 = c128, 2 = f32.
+// dtype: 0 = f64, 1 = c128, 2 = f32, 3 = c64 (row kernel and general
+// kernel; the tiled x-march kernel takes 0-2).
 int launch_axis_stencil(const AxisStencil& st, int dtype, const int n[3],
                         const BC& bc, const void* in, void* out, long nin,
                         int epi, const EpiArgs& ea, hipStream_t stream,

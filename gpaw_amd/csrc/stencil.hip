@@ -341,6 +341,12 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
             typedef typename ScalarOf<T>::type S;
             a.out[gid] = fmac(fmac(acc, (S)a.c0, v), (S)a.pot[gid - b * a.ng],
                               v);
+        } else if constexpr (EPI == EPI_UPDW) {
+            typedef typename WideOf<T>::type WT;
+            const T av = to_narrow(((const WT*)(const void*)a.aux)[gid]);
+            const T r = epi_apply<T, EPI_UPDATE>(acc, v, av, a.c0, a.w, a.s,
+                                                 a.negate, nrm);
+            ((WT*)(void*)a.out)[gid] = to_wide(r);
         } else {
             a.out[gid] = epilogue<T, EPI>(acc, v, a.aux, gid, a.c0, a.w, a.s,
                                           a.negate, nrm);
@@ -653,6 +659,15 @@ int launch_general_stencil(const GeneralStencil& st, int dtype,
         case EPI_PLAIN: return launch_gen_t<float, EPI_PLAIN>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RELAX: return launch_gen_t<float, EPI_RELAX>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_RESID: return launch_gen_t<float, EPI_RESID>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_UPDATE: return launch_gen_t<float, EPI_UPDATE>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_UPDW: return launch_gen_t<float, EPI_UPDW>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        }
+    } else if (dtype == 3) {
+        switch (epi) {
+        case EPI_PLAIN: return launch_gen_t<float2, EPI_PLAIN>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_RESID: return launch_gen_t<float2, EPI_RESID>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_UPDATE: return launch_gen_t<float2, EPI_UPDATE>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_UPDW: return launch_gen_t<float2, EPI_UPDW>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         }
     }
     set_error("general stencil: unsupported dtype %d / epilogue %d", dtype,

@@ -8,6 +8,7 @@ namespace gfd {
 
 template <typename T> struct IsCplx { static constexpr bool value = false; };
 template <> struct IsCplx<double2> { static constexpr bool value = true; };
+template <> struct IsCplx<float2> { static constexpr bool value = true; };
 
 // Bring i into [0, n); multiply the factor by the Bloch phase once per wrap.
 __device__ inline bool wrap_full(int& i, int n, int periodic, double2 hi,
@@ -43,10 +44,34 @@ __device__ inline double2 scale_by<double2>(double2 v, double2 f, bool phased)
 {
     return phased ? cmul(f, v) : v;
 }
+template <>
+__device__ inline float2 scale_by<float2>(float2 v, double2 f, bool phased)
+{
+    return phased ? cmul(make_float2((float)f.x, (float)f.y), v) : v;
+}
 
+// FP64 counterpart of an FP32 data type (EPI_UPDW's aux and out)
+template <typename T> struct WideOf { typedef T type; };
+template <> struct WideOf<float> { typedef double type; };
+template <> struct WideOf<float2> { typedef double2 type; };
+
+__device__ inline float to_narrow(double v) { return (float)v; }
+__device__ inline float2 to_narrow(double2 v)
+{
+    return make_float2((float)v.x, (float)v.y);
+}
+__device__ inline double to_wide(float v) { return (double)v; }
+__device__ inline double2 to_wide(float2 v)
+{
+    return make_double2((double)v.x, (double)v.y);
+}
 __device__ inline double sq(double v) { return v * v; }
 __device__ inline double sq(float v) { return (double)v * (double)v; }
 __device__ inline double sq(double2 v) { return v.x * v.x + v.y * v.y; }
+__device__ inline double sq(float2 v)
+{
+    return (double)v.x * v.x + (double)v.y * v.y;
+}
 
 // Epilogue (see stencil.h) on one point: acc excludes the centre term, v
 // is the centre value, av the auxiliary operand (src or subtrahend).

@@ -19,6 +19,9 @@ struct XEpiArgs {
     double alpha = 0.0;
     void* out2 = nullptr;
     const void* base = nullptr;
+    // FP32 data (dtype 2 / 3) only, K = 2: the input is the FP64
+    // counterpart (f64 / c128), narrowed as it is read
+    int in_wide = 0;
 };
 
 // Per-axis description of one 1-D transfer pass.
@@ -43,7 +46,7 @@ void xfer_force_passes(bool on);
 // Scratch (in elements of the data type) needed for `nin` grids.
 long xfer_scratch_elems(const XferPlan& p, long nin);
 
-// dtype: 0 f64, 1 c128, 2 f32
+// dtype: 0 f64, 1 c128, 2 f32, 3 c64 (c64: K = 2 only)
 int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
                     const void* in, void* out, long nin, void* scratch,
                     const XEpiArgs& ea, hipStream_t stream);

@@ -18,6 +18,8 @@
 // The weights are exact dyadic rationals.
 #include "transfer.h"
 
+#include <type_traits>
+
 namespace gfd {
 
 template <int K> struct W;
@@ -37,6 +39,20 @@ template <> struct W<8> {
 
 template <typename T> struct XIsCplx { static constexpr bool value = false; };
 template <> struct XIsCplx<double2> { static constexpr bool value = true; };
+template <> struct XIsCplx<float2> { static constexpr bool value = true; };
+
+// input value (possibly the wider FP64 type) as the compute type T
+template <typename T, typename TI>
+__device__ inline T narrow(TI v)
+{
+    if constexpr (std::is_same<T, TI>::value) {
+        return v;
+    } else if constexpr (XIsCplx<T>::value) {
+        return T{(float)v.x, (float)v.y};
+    } else {
+        return (T)v;
+    }
+}
 
 template <typename T>
 struct XArgs {
@@ -152,9 +168,9 @@ __global__ void __launch_bounds__(256) k_xfer1d(XArgs<T> a)
 // the caches.  Sums are associated z innermost, then y, then x, as the
 // reference's pass order.
 
-template <typename T>
+template <typename T, typename TI = T>
 struct F3Args {
-    const T* in;
+    const TI* in;
     T* out;
     T* out2;
     const T* base;
@@ -169,9 +185,10 @@ struct F3Args {
     double alpha;
 };
 
-// value of the input at (i0, i1, i2) with per-axis wrap / zero
-template <typename T>
-__device__ inline T f3load(const F3Args<T>& a, const T* __restrict__ g,
+// value of the input at (i0, i1, i2) with per-axis wrap / zero; the
+// phase is applied at the input's precision, then the value is narrowed
+template <typename T, typename TI>
+__device__ inline T f3load(const F3Args<T, TI>& a, const TI* __restrict__ g,
                            int i0, int i1, int i2)
 {
     int ii[3] = {i0, i1, i2};
@@ -192,14 +209,14 @@ __device__ inline T f3load(const F3Args<T>& a, const T* __restrict__ g,
             phased = true;
         }
     }
-    T v = g[((long)ii[0] * a.ni[1] + ii[1]) * a.ni[2] + ii[2]];
-    if constexpr (XIsCplx<T>::value)
-        if (phased) v = cmul(f, v);
-    return v;
+    TI v = g[((long)ii[0] * a.ni[1] + ii[1]) * a.ni[2] + ii[2]];
+    if constexpr (XIsCplx<TI>::value)
+        if (phased) v = apply_phase(v, f);
+    return narrow<T, TI>(v);
 }
 
-template <typename T, int EPI>
-__device__ inline void f3store(const F3Args<T>& a, long gid, T v)
+template <typename T, int EPI, typename TI>
+__device__ inline void f3store(const F3Args<T, TI>& a, long gid, T v)
 {
     typedef typename ScalarOf<T>::type S;
     if constexpr (EPI == XEPI_PLAIN) {
@@ -215,8 +232,8 @@ __device__ inline void f3store(const F3Args<T>& a, long gid, T v)
     }
 }
 
-template <typename T, int EPI>
-__global__ void __launch_bounds__(256) k_restrict2(F3Args<T> a)
+template <typename T, int EPI, typename TI>
+__global__ void __launch_bounds__(256) k_restrict2(F3Args<T, TI> a)
 {
     typedef typename ScalarOf<T>::type S;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
@@ -229,7 +246,7 @@ __global__ void __launch_bounds__(256) k_restrict2(F3Args<T> a)
     const int o0 = (int)(r / a.no[1]);
     const int c0 = 2 * o0 + a.off[0], c1 = 2 * o1 + a.off[1],
               c2 = 2 * o2 + a.off[2];
-    const T* __restrict__ g = a.in + b * a.in_grid;
+    const TI* __restrict__ g = a.in + b * a.in_grid;
     T px[3];
 #pragma unroll
     for (int dx = -1; dx <= 1; dx++) {
@@ -245,11 +262,11 @@ __global__ void __launch_bounds__(256) k_restrict2(F3Args<T> a)
         px[dx + 1] = (S)0.5 * fmac(py[1], (S)0.5, py[2] + py[0]);
     }
     const T v = (S)0.5 * fmac(px[1], (S)0.5, px[2] + px[0]);
-    f3store<T, EPI>(a, gid, v);
+    f3store<T, EPI, TI>(a, gid, v);
 }
 
-template <typename T, int EPI>
-__global__ void __launch_bounds__(256) k_interp2(F3Args<T> a)
+template <typename T, int EPI, typename TI>
+__global__ void __launch_bounds__(256) k_interp2(F3Args<T, TI> a)
 {
     typedef typename ScalarOf<T>::type S;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
@@ -264,7 +281,7 @@ __global__ void __launch_bounds__(256) k_interp2(F3Args<T> a)
     const int c0 = (q0 >> 1) + a.off[0], c1 = (q1 >> 1) + a.off[1],
               c2 = (q2 >> 1) + a.off[2];
     const int n0 = (q0 & 1) + 1, n1 = (q1 & 1) + 1, n2 = (q2 & 1) + 1;
-    const T* __restrict__ g = a.in + b * a.in_grid;
+    const TI* __restrict__ g = a.in + b * a.in_grid;
     T vx[2];
 #pragma unroll
     for (int dx = 0; dx < 2; dx++) {
@@ -281,16 +298,16 @@ __global__ void __launch_bounds__(256) k_interp2(F3Args<T> a)
         vx[dx] = (n1 == 2) ? (S)0.5 * (vy[0] + vy[1]) : vy[0];
     }
     const T v = (n0 == 2) ? (S)0.5 * (vx[0] + vx[1]) : vx[0];
-    f3store<T, EPI>(a, gid, v);
+    f3store<T, EPI, TI>(a, gid, v);
 }
 
-template <typename T, int EPI>
+template <typename T, int EPI, typename TI>
 static int launch_fused2_t(const XferPlan& p, const BC& bc, const void* in,
                            void* out, long nin, const XEpiArgs& ea,
                            hipStream_t s)
 {
-    F3Args<T> a;
-    a.in = (const T*)in;
+    F3Args<T, TI> a;
+    a.in = (const TI*)in;
     a.out = (T*)out;
     a.out2 = (T*)ea.out2;
     a.base = (const T*)ea.base;
@@ -315,25 +332,25 @@ static int launch_fused2_t(const XferPlan& p, const BC& bc, const void* in,
         return ENOTSUP_;
     }
     if (p.interpolate)
-        hipLaunchKernelGGL((k_interp2<T, EPI>), dim3((unsigned)nb), dim3(256),
-                           0, s, a);
+        hipLaunchKernelGGL((k_interp2<T, EPI, TI>), dim3((unsigned)nb),
+                           dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_restrict2<T, EPI>), dim3((unsigned)nb),
+        hipLaunchKernelGGL((k_restrict2<T, EPI, TI>), dim3((unsigned)nb),
                            dim3(256), 0, s, a);
     GFD_LAUNCH_CHECK();
     return OK;
 }
 
-template <typename T>
+template <typename T, typename TI = T>
 static int launch_fused2(const XferPlan& p, const BC& bc, const void* in,
                          void* out, long nin, const XEpiArgs& ea,
                          hipStream_t s)
 {
     switch (ea.epi) {
-    case XEPI_PLAIN: return launch_fused2_t<T, XEPI_PLAIN>(p, bc, in, out, nin, ea, s);
-    case XEPI_NEG: return launch_fused2_t<T, XEPI_NEG>(p, bc, in, out, nin, ea, s);
-    case XEPI_SCALE2: return launch_fused2_t<T, XEPI_SCALE2>(p, bc, in, out, nin, ea, s);
-    case XEPI_SUBFROM: return launch_fused2_t<T, XEPI_SUBFROM>(p, bc, in, out, nin, ea, s);
+    case XEPI_PLAIN: return launch_fused2_t<T, XEPI_PLAIN, TI>(p, bc, in, out, nin, ea, s);
+    case XEPI_NEG: return launch_fused2_t<T, XEPI_NEG, TI>(p, bc, in, out, nin, ea, s);
+    case XEPI_SCALE2: return launch_fused2_t<T, XEPI_SCALE2, TI>(p, bc, in, out, nin, ea, s);
+    case XEPI_SUBFROM: return launch_fused2_t<T, XEPI_SUBFROM, TI>(p, bc, in, out, nin, ea, s);
     }
     set_error("transfer: bad epilogue %d", ea.epi);
     return EINVAL_;
@@ -450,13 +467,30 @@ int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
                     const void* in, void* out, long nin, void* scratch,
                     const XEpiArgs& ea, hipStream_t stream)
 {
-    if (p.K == 2 && !g_xfer_force_passes) {
+    if (ea.in_wide && (p.K != 2 || (dtype != 2 && dtype != 3))) {
+        set_error("transfer: FP64 input only for FP32 data and K = 2");
+        return EINVAL_;
+    }
+    if (dtype == 3 && p.K != 2) {
+        set_error("transfer: complex64 data only for K = 2");
+        return ENOTSUP_;
+    }
+    if (p.K == 2 && (!g_xfer_force_passes || dtype == 3 || ea.in_wide)) {
         if (dtype == 0)
             return launch_fused2<double>(p, bc, in, out, nin, ea, stream);
         if (dtype == 1)
             return launch_fused2<double2>(p, bc, in, out, nin, ea, stream);
         if (dtype == 2)
-            return launch_fused2<float>(p, bc, in, out, nin, ea, stream);
+            return ea.in_wide
+                       ? launch_fused2<float, double>(p, bc, in, out, nin, ea,
+                                                      stream)
+                       : launch_fused2<float>(p, bc, in, out, nin, ea, stream);
+        if (dtype == 3)
+            return ea.in_wide
+                       ? launch_fused2<float2, double2>(p, bc, in, out, nin,
+                                                        ea, stream)
+                       : launch_fused2<float2>(p, bc, in, out, nin, ea,
+                                               stream);
     }
     if (dtype == 0)
         return run_transfer_k<double>(p, bc, in, out, nin, scratch, ea, stream);

@@ -282,13 +282,20 @@ class FDPoissonSolver(BasePoissonSolver):
         _lib.check(status, 'FDPoissonSolver.solve_neutral')
         return niter.value
 
-    def solve_mixed(self, phi, rho, tol=1e-10, levels=4, pre=2, post=2,
+    def solve_mixed(self, phi, rho, tol=1e-10, levels=None, pre=2, post=2,
                     coarse_sweeps=8, w0=-1.0, wc=-1.0, inner=1,
                     max_outer=100):
         """FP32 V-cycles inside FP64 iterative refinement until
         |L phi - rho| / |rho| <= tol (phi is the initial guess and the
-        result).  Returns (refinement steps, final relative residual)."""
+        result).  Returns (refinement steps, final relative residual).
+
+        levels: depth of the FP32 hierarchy (None: all levels of the FP64
+        one).  The coarsest FP32 level gets coarse_sweeps Jacobi sweeps, so
+        a shallow hierarchy leaves smooth error behind and needs more
+        refinement steps."""
         h = self._handle()
+        if levels is None:
+            levels = self.levels + 1
         st = Staging(phi, rho)
         n = _lib.ctypes.c_int(0)
         rel = _lib.ctypes.c_double(0.0)

@@ -4,6 +4,11 @@ gpaw.preconditioner.Preconditioner, gpaw/preconditioner.py:9-93).
 The whole 3-level chain of c/fd_preconditioner.cpp:221-345 runs as one
 native call (gfd_fd_precond) over a block of bands: four transfers and
 three stencil passes with the element-wise steps fused into them.
+
+``fp32=True`` (a keyword the reference does not have) runs the chain in
+FP32 (gfd_fd_precond32): the first restriction reads the FP64 residuals
+and narrows them, the work arrays are float32 / complex64, and the last
+stencil pass writes the FP64 result.
 """
 from __future__ import annotations
 
@@ -16,7 +21,7 @@ from .transformers import Transformer
 
 
 class Preconditioner:
-    def __init__(self, gd0, kin0, dtype=float, block=1, xp=np):
+    def __init__(self, gd0, kin0, dtype=float, block=1, xp=np, fp32=False):
         gd1 = gd0.coarsen()
         gd2 = gd1.coarsen()
         self.gd0, self.gd1, self.gd2 = gd0, gd1, gd2
@@ -36,22 +41,28 @@ class Preconditioner:
         self.interpolator1 = self.interpolator_object1.apply
         self.use_c_precond = True
         self.xp = xp
+        self.fp32 = bool(fp32)
         self._scratch = None
 
     def calculate_kinetic_energy(self, psit_xG, kpt):
         return None
 
     def _work(self, nb):
-        if self._scratch is None or self._scratch[0].shape[1] < nb:
-            tdt = torch.complex128 if self.dtype == complex else torch.float64
+        """(fine q0, level-1 r1 d1 q1, level-2 r2 d2 q2) of at least nb
+        bands."""
+        if self._scratch is None or self._scratch[1].shape[1] < nb:
+            cplx = self.dtype == complex
+            f64 = torch.complex128 if cplx else torch.float64
+            wdt = (torch.complex64 if cplx else torch.float32) \
+                if self.fp32 else f64
             dev = device()
             n = max(nb, self.block)
             self._scratch = (
-                torch.zeros((2, n) + tuple(self.gd0.n_c), dtype=tdt,
+                torch.zeros((n,) + tuple(self.gd0.n_c), dtype=wdt,
                             device=dev),
-                torch.zeros((3, n) + tuple(self.gd1.n_c), dtype=tdt,
+                torch.zeros((3, n) + tuple(self.gd1.n_c), dtype=wdt,
                             device=dev),
-                torch.zeros((3, n) + tuple(self.gd2.n_c), dtype=tdt,
+                torch.zeros((3, n) + tuple(self.gd2.n_c), dtype=wdt,
                             device=dev))
         return self._scratch
 
@@ -74,14 +85,13 @@ class Preconditioner:
                             'and q0.')
         phases = getattr(kpt, 'phase_cd', None) if kpt is not None else None
         s0, s1, s2 = self._work(nb)
-        q0 = s0[0, :nb]
+        q0 = s0[:nb]
         r1, d1, q1 = s1[0, :nb], s1[1, :nb], s1[2, :nb]
         r2, d2, q2 = s2[0, :nb], s2[1, :nb], s2[2, :nb]
         st = Staging(residuals)
         res = st.dev[0].contiguous()
         if out is None:
-            d0 = s0[1, :nb] if not is_device(residuals) else \
-                torch.empty_like(res)
+            d0 = torch.empty_like(res)
         else:
             d0 = out if is_device(out) else torch.empty_like(res)
         keep, ph = (None, None)
@@ -89,7 +99,8 @@ class Preconditioner:
             keep, ph = _lib.phases_arg(np.ones((3, 2), complex)
                                        if phases is None else phases)
         lib = _lib.load()
-        _lib.check(lib.gfd_fd_precond(
+        fn = lib.gfd_fd_precond32 if self.fp32 else lib.gfd_fd_precond
+        _lib.check(fn(
             self.restrictor_object0._handle(),
             self.restrictor_object1._handle(),
             self.interpolator_object1._handle(),

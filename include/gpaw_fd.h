@@ -109,6 +109,19 @@ int gfd_fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
                    void* d2, void* q2, const void* residuals, long nin,
                    double step, const double* phases, void* stream);
 
+/* gfd_fd_precond with the multigrid chain in FP32: the work arrays q0, r1,
+ * d1, q1, r2, d2, q2 are float32 (complex64 for complex objects) of the
+ * same shapes; residuals and d0 stay float64 / complex128.  Every pass
+ * computes and accumulates in FP32.  nn=1 transformers only.  No
+ * counterpart in the reference (its preconditioner is FP64 only). */
+int gfd_fd_precond32(gfd_transformer* rest0, gfd_transformer* rest1,
+                     gfd_transformer* intp1, gfd_transformer* intp2,
+                     gfd_operator* kin0, gfd_operator* kin1,
+                     gfd_operator* kin2, void* d0, void* q0, void* r1,
+                     void* d1, void* q1, void* r2, void* d2, void* q2,
+                     const void* residuals, long nin, double step,
+                     const double* phases, void* stream);
+
 /* ---------------------------------------------------------------- Poisson
  * Native multigrid driver of FDPoissonSolver (gpaw/poisson.py:321-537):
  * ops[0..nlevels] are the Laplacians of levels 0..L, restrictors[l] /
@@ -153,6 +166,56 @@ int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi, const double* rho,
                             int coarse_sweeps, double w0, double wc,
                             int inner, int max_outer, int* n_outer,
                             double* relres, void* stream);
+
+/* --------------------------------------------------------- batched grids
+ * Many independent real Poisson problems of uneven shape (no counterpart
+ * in the reference, which solves one grid at a time; the per-grid algebra
+ * is FDPoissonSolver's, gpaw/poisson.py:366-537).  Grid g, level l of the
+ * FP32 hierarchy: ops[g * L + l] (real axis-only stencils, radius <= 3),
+ * rst / itp[g * (L - 1) + l] (real nn=1 transformers), weights[l] the
+ * Jacobi weight of level l.  Level arrays are the grids' arrays
+ * concatenated in grid order. */
+typedef struct gfd_batch_poisson gfd_batch_poisson;
+
+int gfd_batch_poisson_new(int G, int L, gfd_operator** ops,
+                          gfd_transformer** rst, gfd_transformer** itp,
+                          const double* weights, gfd_batch_poisson** out);
+void gfd_batch_poisson_free(gfd_batch_poisson* b);
+
+/* Points of all grids at one level (-1 for a bad level). */
+long gfd_batch_poisson_level_points(gfd_batch_poisson* b, int level);
+
+/* Time candidate tile shapes per level on this batch and keep the fastest
+ * (done by the first apply / transfer / solve if never called). */
+int gfd_batch_poisson_tune(gfd_batch_poisson* b, void* stream);
+
+/* Chosen tile of a level (-1: the FP64 fine-level residual): threads along
+ * z and y, z points per thread, resident blocks (0: one block per tile). */
+int gfd_batch_poisson_tiles(gfd_batch_poisson* b, int level, int* bz,
+                            int* by, int* pz, long* blocks);
+
+/* out = L_level in on every grid, float64 (fp32 = 0) or float32 level
+ * arrays; one launch. */
+int gfd_batch_poisson_apply(gfd_batch_poisson* b, int level, int fp32,
+                            const void* in, void* out, void* stream);
+
+/* Restriction level -> level + 1 (interpolate = 0), or
+ * out = base - I(in) from level + 1 to level; one launch. */
+int gfd_batch_poisson_transfer(gfd_batch_poisson* b, int level,
+                               int interpolate, int fp32, const void* in,
+                               void* out, const void* base, void* stream);
+
+/* FP32 V-cycles (pre / post Jacobi sweeps, coarse_sweeps on the coarsest
+ * level) inside FP64 refinement until |L phi_g - rho_g| / |rho_g| <=
+ * tol_rel for every grid; phis[g] in/out and rhos[g] are float64 device
+ * arrays of grid g's fine shape.  use_graph replays one captured
+ * refinement step.  relres[g] receives the final relative residuals;
+ * GFD_ECONVERGE after max_outer steps. */
+int gfd_batch_poisson_solve_mixed(gfd_batch_poisson* b, double* const* phis,
+                                  const double* const* rhos, double tol_rel,
+                                  int pre, int post, int coarse_sweeps,
+                                  int max_outer, int use_graph, int* n_outer,
+                                  double* relres, void* stream);
 
 /* ---------------------------------------------------------------- bands
  * FD eigensolver step (complex128 bands, row-major nbands x npts with row

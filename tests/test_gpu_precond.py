@@ -50,3 +50,27 @@ def test_blocks_and_host_arrays():
     assert relerr(out_h, ref) < TOL
     one = host(P(dev(res[2]), K(ph)))
     assert relerr(one, ref[2]) < TOL
+
+
+@pytest.mark.parametrize('dtype,pbc', [(complex, True), (float, True),
+                                       (float, False)])
+def test_fp32_chain(dtype, pbc):
+    """FP32 chain against the FP64 oracle: float32 rounding over ~10
+    passes, so a relative max-norm error of a few 1e-7 (bar 2e-6)."""
+    rng = np.random.default_rng(6)
+    gd = GridDescriptor((32, 24, 48), (5.0, 4.0, 7.0), pbc)
+    P = Preconditioner(gd, Laplace(gd, -0.5, 3, dtype), dtype, block=4,
+                       fp32=True)
+    res = rand(rng, (6,) + tuple(gd.n_c), dtype)
+    ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+    ref = oracle.fd_precond(spec(P.kin0), spec(P.kin1), spec(P.kin2),
+                            tspec(P.restrictor_object0),
+                            tspec(P.restrictor_object1),
+                            tspec(P.interpolator_object1),
+                            tspec(P.interpolator_object2), res, P.step, ph)
+    say('precond fp32', dtype.__name__, pbc)
+    out = host(P(dev(res), K(ph)))
+    assert out.dtype == res.dtype
+    err = relerr(out, ref)
+    assert err < 2e-6
+    assert err > 1e-12        # it did run in FP32
