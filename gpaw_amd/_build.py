@@ -19,7 +19,7 @@ ARCH = 'gfx950'
 # longest compiles first (the pool runs them in this order)
 SOURCES = ['rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
            'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
-           'stencil_rows.hip', 'vector.hip', 'bands.hip']
+           'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',

@@ -57,6 +57,9 @@ SIGNATURES = {
                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                    _dp, ctypes.POINTER(_vp)]),
     'gfd_batch_poisson_free': (None, [_vp]),
+    'gfd_batch_poisson_set_coarse': (_i, [_vp, _ip, ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp), _dp]),
     'gfd_batch_poisson_level_points': (_l, [_vp, _i]),
     'gfd_batch_poisson_tune': (_i, [_vp, _vp]),
     'gfd_batch_poisson_tiles': (_i, [_vp, _i, _ip, _ip, _ip,

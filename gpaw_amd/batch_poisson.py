@@ -6,7 +6,10 @@ transfers, Jacobi weights 2/3 and 1).  The batched solve runs FP32 V-cycles
 over every grid at once -- one persistent launch per stencil or transfer
 stage of a level -- inside FP64 residual refinement, with one refinement
 step captured as a hipGraph.  Tile shapes are timed per level on the actual
-batch the first time the batch runs (``tune``).
+batch the first time the batch runs (``tune``).  With ``coarse=True``
+each grid's levels from the coarsest batched one down to its own coarsest
+(GPAW coarsens while every side is even, so depths differ between grids)
+run in one LDS-resident workgroup per grid, a single launch per V-cycle.
 
 There is no counterpart in the reference, which solves one grid at a time.
 """
@@ -20,9 +23,10 @@ from .poisson import FDPoissonSolver, PoissonConvergenceError
 
 
 class BatchedPoissonSolver:
-    def __init__(self, nn=3, levels=4):
+    def __init__(self, nn=3, levels=4, coarse=True):
         self.nn = nn
         self.levels = levels
+        self.coarse = coarse
         self.solvers = []
         self._native = None
 
@@ -56,7 +60,31 @@ class BatchedPoissonSolver:
                 G, L, ops, rst, itp, w, _lib.ctypes.byref(h)),
                 'BatchedPoissonSolver')
             self._native = h
+            if self.coarse:
+                self._set_coarse(True)
         return self._native
+
+    def _set_coarse(self, on):
+        lib = _lib.load()
+        V = _lib.ctypes.c_void_p
+        if not on:
+            _lib.check(lib.gfd_batch_poisson_set_coarse(
+                self._native, None, None, None, None, None), 'set_coarse')
+            return
+        nlev = [s.levels + 1 for s in self.solvers]
+        ops = [op._handle().value for s in self.solvers
+               for op in s.operators]
+        rst = [t._handle().value for s in self.solvers
+               for t in s.restrictors] or [None]
+        itp = [t._handle().value for s in self.solvers
+               for t in s.interpolators] or [None]
+        deep = max(self.solvers, key=lambda s: s.levels)
+        _lib.check(lib.gfd_batch_poisson_set_coarse(
+            self._native, (_lib.ctypes.c_int * len(nlev))(*nlev),
+            (V * len(ops))(*ops), (V * len(rst))(*rst),
+            (V * len(itp))(*itp),
+            (_lib.ctypes.c_double * len(deep.weights))(*deep.weights)),
+            'set_coarse')
 
     def _free(self):
         if self._native is not None:
@@ -152,6 +180,12 @@ class BatchedPoissonSolver:
             int(coarse_sweeps), int(max_outer), int(bool(graph)),
             _lib.ctypes.byref(n), rel, stream_handle())
         relres = np.array(rel[:])
+        if status == _lib.GFD_ENOTSUP and self.coarse:
+            # some grid's coarse levels exceed one workgroup's LDS
+            self.coarse = False
+            self._set_coarse(False)
+            return self.solve_mixed(phis, rhos, tol, pre, post,
+                                    coarse_sweeps, max_outer, graph)
         if status == _lib.GFD_ECONVERGE:
             raise PoissonConvergenceError(
                 'batched mixed solve: max relative residual %.3g after %d '

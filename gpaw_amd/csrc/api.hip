@@ -17,6 +17,7 @@
 #include "vector.h"
 #include "bands.h"
 #include "handles.h"
+#include "coarse.h"
 
 namespace gfd {
 
@@ -380,7 +381,7 @@ extern "C" int gfd_operator_relax(gfd_operator* op, int method, double* f,
 static int residual_norm(gfd_operator* op, int dtype, const void* in,
                          const void* b, void* r, long nin,
                          const double* phases, double* dev_norm,
-                         hipStream_t s)
+                         hipStream_t s, bool narrow = false)
 {
     EpiArgs ea{};
     ea.aux = b;
@@ -391,7 +392,8 @@ static int residual_norm(gfd_operator* op, int dtype, const void* in,
         ea.partial = (double*)op->partial.p;
     }
     int nb = 0;
-    GFD_TRY(op_pass(op, dtype, in, r, nin, phases, EPI_RESID, ea, s, &nb));
+    GFD_TRY(op_pass(op, dtype, in, r, nin, phases,
+                    narrow ? EPI_RESN : EPI_RESID, ea, s, &nb));
     if (dev_norm) GFD_TRY(launch_sum_partials(ea.partial, nb, dev_norm, s));
     return OK;
 }
@@ -698,6 +700,20 @@ struct Levels {
     std::vector<void*> phi, tmp, rho, res;
 };
 
+// Coarse levels of a schedule run by the LDS-resident workgroup.
+struct CoarseSolve {
+    bool valid = false;
+    int lc = -1;      // first level in LDS (-1: none)
+    size_t lds = 0;
+    CGrid host{};
+    DevBuf dev;
+};
+
+// LDS-resident coarse solve on (default) / off, and the largest first
+// level (points) it takes
+static bool g_coarse_lds = true;
+static long g_coarse_max_pts = 4096;
+
 struct gfd_poisson {
     int L = 0;  // number of coarsenings
     std::vector<gfd_operator*> ops;
@@ -716,6 +732,7 @@ struct gfd_poisson {
     hipGraphExec_t g32 = nullptr;
     int g32_key[6] = {-1, -1, -1, -1, -1, -1};
     double g32_w[2] = {0, 0};
+    CoarseSolve cs64, cs32;
 };
 
 static int alloc_levels(gfd_poisson* p, Levels& lv, int dtype, int L)
@@ -862,6 +879,97 @@ static double sched_w(const Schedule& sc, int l)
     return sc.w[l];
 }
 
+// Plan the LDS-resident solve of levels [lc, coarsest] for a schedule: the
+// first lc >= 1 whose level arrays fit one workgroup's LDS (rhs of level lc
+// stays in global memory).  *changed reports a new plan (graphs that
+// captured the old one must be dropped).  Not callable under capture.
+static int plan_coarse(gfd_poisson* p, const Schedule& sc, int dtype,
+                       CoarseSolve& c, hipStream_t s, bool* changed)
+{
+    const size_t es = dtype == 2 ? 4 : 8;
+    int lc = -1;
+    size_t bytes = 0;
+    for (int l0 = 1; g_coarse_lds && l0 <= sc.coarsest; l0++) {
+        if (sc.coarsest - l0 + 1 > CMAXLEV) continue;
+        if (p->ops[l0]->ng() > g_coarse_max_pts) continue;
+        size_t b = 0;
+        bool ok = true;
+        for (int l = l0; l <= sc.coarsest && ok; l++) {
+            const gfd_operator* op = p->ops[l];
+            ok = op->axis_ok && op->real && op->ax.R <= CMAXR;
+            if (l < sc.coarsest)
+                ok = ok && p->rst[l]->plan.K == 2 && p->itp[l]->plan.K == 2;
+            b += es * op->ng() * (l == l0 ? 2 : 3);
+        }
+        if (ok && b <= COARSE_LDS_MAX) {
+            lc = l0;
+            bytes = b;
+            break;
+        }
+    }
+    // stage the first level's rhs in LDS too when it fits
+    const bool r_lds =
+        lc >= 0 && bytes + es * p->ops[lc]->ng() <= COARSE_LDS_MAX;
+    if (r_lds) bytes += es * p->ops[lc]->ng();
+    CGrid g{};
+    if (lc >= 0) {
+        g.nlev = sc.coarsest - lc + 1;
+        int off = 0;
+        for (int i = 0; i < g.nlev; i++) {
+            const int l = lc + i;
+            const gfd_operator* op = p->ops[l];
+            CLevel& L = g.lev[i];
+            for (int d = 0; d < 3; d++) {
+                L.n[d] = op->n[d];
+                L.periodic[d] = op->periodic[d];
+                L.has[d] = op->ax.has[d];
+                for (int k = 0; k < CMAXR; k++) {
+                    L.cp[d][k] = k < op->ax.R ? op->ax.cp[d][k] : 0.0;
+                    L.cm[d][k] = k < op->ax.R ? op->ax.cm[d][k] : 0.0;
+                }
+            }
+            L.R = op->ax.R;
+            L.c0 = centre_coef(op);
+            L.w = sched_w(sc, l);
+            L.pre = sched_pre(sc, l);
+            L.post = sched_post(sc, l);
+            const int n = (int)op->ng();
+            L.a = off;
+            off += n;
+            L.b = off;
+            off += n;
+            if (i == 0 && !r_lds) {
+                L.r = -1;
+            } else {
+                L.r = off;
+                off += n;
+            }
+            if (l < sc.coarsest)
+                for (int d = 0; d < 3; d++) {
+                    L.roff[d] = p->rst[l]->plan.ax[d].off;
+                    L.ioff[d] = p->itp[l]->plan.ax[d].off;
+                    L.iskip0[d] = p->itp[l]->plan.ax[d].skip0;
+                }
+        }
+        g.lds_elems = off;
+    }
+    const bool same = c.valid && c.lc == lc &&
+                      (lc < 0 || std::memcmp(&c.host, &g, sizeof g) == 0);
+    if (changed) *changed = !same;
+    if (same) return OK;
+    if (lc >= 0) {
+        GFD_TRY(c.dev.ensure(sizeof(CGrid), s));
+        GFD_CHECK_HIP(hipMemcpyAsync(c.dev.p, &g, sizeof(CGrid),
+                                     hipMemcpyHostToDevice, s));
+        GFD_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    c.host = g;
+    c.lc = lc;
+    c.lds = bytes;
+    c.valid = true;
+    return OK;
+}
+
 // gpaw/poisson.py:500-526 for one level; *cur / *alt hold phi_l.
 static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
                   void** cur, void** alt, const void* rho, hipStream_t s)
@@ -882,8 +990,15 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
                         xe, s));
         void* c1 = lv.phi[l + 1];
         void* a1 = lv.tmp[l + 1];
-        GFD_TRY(vec_zero_bytes(c1, es * p->ops[l + 1]->ng(), s));
-        GFD_TRY(vcycle(p, lv, sc, l + 1, &c1, &a1, lv.rho[l + 1], s));
+        const CoarseSolve& cs = dt == 2 ? p->cs32 : p->cs64;
+        if (cs.valid && cs.lc == l + 1) {
+            // levels l+1 .. coarsest in one LDS-resident workgroup
+            GFD_TRY(launch_coarse((const CGrid*)cs.dev.p, 1, cs.lds, dt,
+                                  lv.rho[l + 1], c1, s));
+        } else {
+            GFD_TRY(vec_zero_bytes(c1, es * p->ops[l + 1]->ng(), s));
+            GFD_TRY(vcycle(p, lv, sc, l + 1, &c1, &a1, lv.rho[l + 1], s));
+        }
         XEpiArgs ce;
         ce.epi = XEPI_SUBFROM;
         ce.base = *cur;
@@ -973,6 +1088,7 @@ static int read_scalar(gfd_poisson* p, hipStream_t s, double* h)
 static int poisson_iterate(gfd_poisson* p, double* phi, const double* rho,
                            double* error, hipStream_t s)
 {
+    GFD_TRY(plan_coarse(p, ref_schedule(p), 0, p->cs64, s, nullptr));
     GFD_TRY(p->scalar.ensure(4 * sizeof(double), s));
     GFD_TRY(vcycle64_norm(p, phi, rho, (double*)p->scalar.p, s));
     double h = 0.0;
@@ -1042,6 +1158,14 @@ extern "C" int gfd_poisson_solve(gfd_poisson* p, double* phi,
     double* dn = (double*)p->scalar.p;
     int it = 1;
     double err = 0.0;
+    {
+        bool changed = false;
+        GFD_TRY(plan_coarse(p, ref_schedule(p), 0, p->cs64, s, &changed));
+        if (changed && p->g64) {
+            (void)hipGraphExecDestroy(p->g64);
+            p->g64 = nullptr;
+        }
+    }
     if (p->use_graph) {
         GFD_TRY(enter_owned(p, phi, rho, s));
         double* phi0 = (double*)p->lv.phi[0];
@@ -1090,25 +1214,24 @@ extern "C" int gfd_poisson_set_graph(gfd_poisson* p, int on)
 }
 
 // One refinement step of the mixed-precision solve on the owned buffers:
-// r32 = (float) r64; e = 0; `inner` f32 V-cycles on L e = r32;
-// phi -= e; r64 = L phi - rho with its squared norm into *dn.
+// e = 0; `inner` f32 V-cycles on L e = r32; phi -= e; r32 = (float)(L phi
+// - rho) with the FP64 squared norm into *dn.
 static int mixed_step(gfd_poisson* p, const Schedule& sc, int inner,
                       double* dn, hipStream_t s)
 {
     const long ng = p->ops[0]->ng();
     double* phi0 = (double*)p->lv.phi[0];
     const double* rho0 = (const double*)p->lv.rho[0];
-    double* r64 = (double*)p->lv.res[0];
     Levels& lv = p->lv32;
     float* r32 = (float*)lv.rho[0];
-    GFD_TRY(vec_d2f(r64, r32, ng, s));
     void* cur = lv.phi[0];
     void* alt = lv.tmp[0];
     GFD_TRY(vec_zero_bytes(cur, 4 * ng, s));
     for (int c = 0; c < inner; c++)
         GFD_TRY(vcycle(p, lv, sc, 0, &cur, &alt, r32, s));
     GFD_TRY(vec_sub_f(phi0, (const float*)cur, ng, s));
-    return residual_norm(p->ops[0], 0, phi0, rho0, r64, 1, nullptr, dn, s);
+    return residual_norm(p->ops[0], 0, phi0, rho0, r32, 1, nullptr, dn, s,
+                         true);
 }
 
 extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
@@ -1136,8 +1259,11 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
     sc.coarse_sweeps = coarse_sweeps;
     sc.w0 = w0;
     sc.wc = wc;
+    bool cs_changed = false;
+    GFD_TRY(plan_coarse(p, sc, 2, p->cs32, s, &cs_changed));
     const int key[6] = {L32, pre, post, coarse_sweeps, inner, 0};
-    bool same = p->g32 && p->g32_w[0] == w0 && p->g32_w[1] == wc;
+    bool same = p->g32 && !cs_changed && p->g32_w[0] == w0 &&
+                p->g32_w[1] == wc;
     for (int i = 0; i < 6; i++) same = same && p->g32_key[i] == key[i];
     if (!same && p->g32) {
         (void)hipGraphExecDestroy(p->g32);
@@ -1153,8 +1279,8 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
         sizeof(double) * (vec_dot_partials(ng) + 1), p->gs));
     GFD_TRY(vec_dot(rho0, rho0, ng, (double*)p->ops[0]->partial.p, dn, p->gs));
     GFD_TRY(read_scalar(p, p->gs, &rn2));
-    GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, p->lv.res[0], 1, nullptr,
-                          dn, p->gs));
+    GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, p->lv32.rho[0], 1,
+                          nullptr, dn, p->gs, true));
     GFD_TRY(read_scalar(p, p->gs, &err2));
     const double tol2 = tol_rel * tol_rel * rn2;
     int outer = 0;
@@ -1306,6 +1432,14 @@ extern "C" int gfd_set_tuning(const char* key, long value)
             return EINVAL_;
         }
         g_force_path = (int)value;
+        return OK;
+    }
+    if (std::strcmp(key, "coarse_lds") == 0) {
+        g_coarse_lds = value != 0;
+        return OK;
+    }
+    if (std::strcmp(key, "coarse_max_points") == 0) {
+        g_coarse_max_pts = value > 0 ? value : 4096;
         return OK;
     }
     if (std::strcmp(key, "xfer_passes") == 0) {

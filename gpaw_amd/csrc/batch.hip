@@ -21,6 +21,7 @@
 #include <cmath>
 #include <vector>
 
+#include "coarse.h"
 #include "gfd_common.h"
 #include "handles.h"
 #include "stencil_dev.h"
@@ -349,6 +350,17 @@ struct gfd_batch_poisson {
     hipGraphExec_t graph = nullptr;
     int gkey[4] = {-1, -1, -1, -1};
     int num_cu = 256;
+    // per-grid levels L-1 .. nlev-1 in LDS-resident workgroups (one per
+    // grid) in place of the coarsest batched level
+    bool coarse = false;
+    std::vector<int> nlev;
+    std::vector<std::vector<gfd_operator*>> cops;
+    std::vector<std::vector<gfd_transformer*>> crst, citp;
+    std::vector<double> cw;
+    std::vector<CGrid> cgrids;
+    size_t clds = 0;
+    int ckey[3] = {-1, -1, -1};
+    DevBuf d_cgrids;
 };
 
 namespace gfd {
@@ -691,12 +703,77 @@ extern "C" void gfd_batch_poisson_free(gfd_batch_poisson* b)
     }
     b->phi.release();
     b->rho.release();
+    b->d_cgrids.release();
     b->partial.release();
     b->norms.release();
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
     if (b->gs) (void)hipStreamDestroy(b->gs);
     delete b;
+}
+
+extern "C" int gfd_batch_poisson_set_coarse(gfd_batch_poisson* b,
+                                            const int* nlev,
+                                            gfd_operator** ops,
+                                            gfd_transformer** rst,
+                                            gfd_transformer** itp,
+                                            const double* weights)
+{
+    if (!b || (nlev && (!ops || !weights))) {
+        set_error("gfd_batch_poisson_set_coarse: bad arguments");
+        return EINVAL_;
+    }
+    b->coarse = false;
+    b->ckey[0] = b->ckey[1] = b->ckey[2] = -1;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    if (!nlev) return OK;
+    const int lc = b->L - 1;
+    b->nlev.assign(nlev, nlev + b->G);
+    b->cops.assign(b->G, {});
+    b->crst.assign(b->G, {});
+    b->citp.assign(b->G, {});
+    int maxlev = 0;
+    long o = 0, t = 0;
+    for (int g = 0; g < b->G; g++) {
+        const int n = nlev[g];
+        if (n < b->L || n - lc > CMAXLEV || (n > 1 && (!rst || !itp))) {
+            set_error("batched Poisson: grid %d has %d levels (need %d .. "
+                      "%d)",
+                      g, n, b->L, lc + CMAXLEV);
+            return EINVAL_;
+        }
+        maxlev = std::max(maxlev, n);
+        for (int l = 0; l < n; l++) {
+            gfd_operator* op = ops[o + l];
+            if (!op || !op->real || !op->axis_ok || op->ax.R > CMAXR) {
+                set_error("batched Poisson: grid %d level %d needs a real "
+                          "axis-only stencil",
+                          g, l);
+                return EINVAL_;
+            }
+            b->cops[g].push_back(op);
+        }
+        for (int l = 0; l + 1 < n; l++) {
+            gfd_transformer* r = rst[t + l];
+            gfd_transformer* i = itp[t + l];
+            if (!r || !i || r->plan.K != 2 || i->plan.K != 2) {
+                set_error("batched Poisson: grid %d level %d needs nn=1 "
+                          "transfers",
+                          g, l);
+                return EINVAL_;
+            }
+            b->crst[g].push_back(r);
+            b->citp[g].push_back(i);
+        }
+        o += n;
+        t += n - 1;
+    }
+    b->cw.assign(weights, weights + maxlev);
+    b->coarse = true;
+    return OK;
 }
 
 extern "C" long gfd_batch_poisson_level_points(gfd_batch_poisson* b,
@@ -783,7 +860,93 @@ static int b_vcycle(gfd_batch_poisson* b, int l, void** cur, void** alt,
         std::swap(*cur, *alt);
         return b_relax(b, l, cur, alt, rhs, post, s);
     }
+    if (b->coarse)
+        return launch_coarse((const CGrid*)b->d_cgrids.p, b->G, b->clds, 2,
+                             rhs, *cur, s);
     return b_relax(b, l, cur, alt, rhs, coarse, s);
+}
+
+// CGrid of every grid for levels L-1 .. nlev_g - 1 (pre / post sweeps,
+// `coarse` sweeps on each grid's coarsest level); uploaded when changed.
+static int b_plan_coarse(gfd_batch_poisson* b, int pre, int post, int coarse,
+                         hipStream_t s, bool* changed)
+{
+    *changed = false;
+    if (!b->coarse) return OK;
+    const int key[3] = {pre, post, coarse};
+    if (std::equal(key, key + 3, b->ckey)) return OK;
+    const int lc = b->L - 1;
+    b->cgrids.assign(b->G, CGrid{});
+    size_t maxb = 0;
+    for (int g = 0; g < b->G; g++) {
+        CGrid& cg = b->cgrids[g];
+        cg.nlev = b->nlev[g] - lc;
+        int off = 0;
+        for (int i = 0; i < cg.nlev; i++) {
+            const int l = lc + i;
+            const gfd_operator* op = b->cops[g][l];
+            CLevel& L = cg.lev[i];
+            for (int d = 0; d < 3; d++) {
+                L.n[d] = op->n[d];
+                L.periodic[d] = op->periodic[d];
+                L.has[d] = op->ax.has[d];
+                for (int k = 0; k < CMAXR; k++) {
+                    L.cp[d][k] = k < op->ax.R ? op->ax.cp[d][k] : 0.0;
+                    L.cm[d][k] = k < op->ax.R ? op->ax.cm[d][k] : 0.0;
+                }
+            }
+            L.R = op->ax.R;
+            L.c0 = op->ax.c0;
+            L.w = b->cw[l];
+            L.pre = pre;
+            L.post = i + 1 == cg.nlev ? coarse : post;
+            const int n = (int)op->ng();
+            L.a = off;
+            off += n;
+            L.b = off;
+            off += n;
+            L.r = off;  // first level: staged, dropped below if no room
+            off += n;
+            if (i + 1 < cg.nlev)
+                for (int d = 0; d < 3; d++) {
+                    L.roff[d] = b->crst[g][l]->plan.ax[d].off;
+                    L.ioff[d] = b->citp[g][l]->plan.ax[d].off;
+                    L.iskip0[d] = b->citp[g][l]->plan.ax[d].skip0;
+                }
+        }
+        if ((size_t)off * 4 > COARSE_LDS_MAX) {
+            // no room to stage the first level's rhs: read it from global
+            // memory in every pass (it sits last in the layout)
+            const int n0 = cg.lev[0].n[0] * cg.lev[0].n[1] * cg.lev[0].n[2];
+            const int r0 = cg.lev[0].r;
+            for (int i = 1; i < cg.nlev; i++) {
+                CLevel& L = cg.lev[i];
+                if (L.a > r0) L.a -= n0;
+                if (L.b > r0) L.b -= n0;
+                if (L.r > r0) L.r -= n0;
+            }
+            cg.lev[0].r = -1;
+            off -= n0;
+        }
+        cg.lds_elems = off;
+        cg.rhs_off = b->lev[lc].st[g].off;
+        cg.out_off = b->lev[lc].st[g].off;
+        maxb = std::max(maxb, (size_t)off * 4);
+    }
+    if (maxb > COARSE_LDS_MAX) {
+        set_error("batched Poisson: coarse levels need %zu bytes of LDS",
+                  maxb);
+        return ENOTSUP_;
+    }
+    GFD_TRY(b->d_cgrids.ensure(sizeof(CGrid) * b->G, s));
+    GFD_CHECK_HIP(hipMemcpyAsync(b->d_cgrids.p, b->cgrids.data(),
+                                 sizeof(CGrid) * b->G, hipMemcpyHostToDevice,
+                                 s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    b->clds = maxb;
+    std::copy(key, key + 3, b->ckey);
+    *changed = true;
+    return OK;
 }
 
 // One refinement step: e = V(r32), phi -= e, r = L phi - rho (FP64, per-grid
@@ -943,8 +1106,10 @@ extern "C" int gfd_batch_poisson_solve_mixed(
             if (err2[g] > tol_rel * tol_rel * rn2[g]) return false;
         return true;
     };
-    const int key[4] = {pre, post, coarse_sweeps, 0};
-    if (b->graph && !std::equal(key, key + 4, b->gkey)) {
+    bool cchanged = false;
+    GFD_TRY(b_plan_coarse(b, pre, post, coarse_sweeps, s, &cchanged));
+    const int key[4] = {pre, post, coarse_sweeps, b->coarse ? 1 : 0};
+    if (b->graph && (cchanged || !std::equal(key, key + 4, b->gkey))) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }

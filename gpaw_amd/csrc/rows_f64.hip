@@ -14,6 +14,9 @@ int row_launch_f64(const AxisStencil& st, const XmTile& tl, const int n[3],
     case EPI_RELAX:
         return launch_xr_r<double, EPI_RELAX>(st, tl, n, bc, in, out, nin, ea,
                                          stream, nblocks_out);
+    case EPI_RESN:
+        return launch_xr_r<double, EPI_RESN>(st, tl, n, bc, in, out, nin, ea,
+                                             stream, nblocks_out);
     case EPI_RESID:
         return launch_xr_r<double, EPI_RESID>(st, tl, n, bc, in, out, nin, ea,
                                          stream, nblocks_out);

@@ -314,6 +314,16 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                         *reinterpret_cast<CW*>((WT*)(void*)a.out + idx) = rw;
                         continue;
                     }
+                    if constexpr (EPI == EPI_RESN) {
+                        const C an = *reinterpret_cast<const C*>(a.aux + idx);
+                        float* o = (float*)(void*)a.out + idx;
+#pragma unroll
+                        for (int p = 0; p < PZ; p++)
+                            o[p] = (float)epi_apply<T, EPI_RESID>(
+                                acc[p], q[u][j].v[p], an.v[p], a.c0, a.w,
+                                a.s, a.negate, nrm);
+                        continue;
+                    }
                     C av = zero_chunk<T, PZ>();
                     if constexpr (EPI != EPI_PLAIN && EPI != EPI_POT)
                         av = *reinterpret_cast<const C*>(a.aux + idx);
@@ -338,7 +348,8 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
             }
         }
     }
-    if (EPI == EPI_RESID && a.partial) block_sum_store(nrm, a.partial, L);
+    if ((EPI == EPI_RESID || EPI == EPI_RESN) && a.partial)
+        block_sum_store(nrm, a.partial, L);
 }
 
 template <typename T> struct PzOf { static constexpr int value = 16 / sizeof(T); };

@@ -18,6 +18,7 @@ enum Epi : int {
     EPI_UPDATE = 3,  // out = +-(v - s ((c0 v + acc) - aux))
     EPI_POT = 4,     // out = (c0 v + acc) + pot v, pot real per grid point
     EPI_UPDW = 5,    // EPI_UPDATE on FP32 data with aux and out in FP64
+    EPI_RESN = 6,    // EPI_RESID on FP64 data, out narrowed to FP32
                      // (shared by the grids of a batch): H psi
 };
 

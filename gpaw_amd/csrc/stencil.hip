@@ -341,6 +341,9 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
             typedef typename ScalarOf<T>::type S;
             a.out[gid] = fmac(fmac(acc, (S)a.c0, v), (S)a.pot[gid - b * a.ng],
                               v);
+        } else if constexpr (EPI == EPI_RESN) {
+            ((float*)(void*)a.out)[gid] = (float)epilogue<T, EPI_RESID>(
+                acc, v, a.aux, gid, a.c0, a.w, a.s, a.negate, nrm);
         } else if constexpr (EPI == EPI_UPDW) {
             typedef typename WideOf<T>::type WT;
             const T av = to_narrow(((const WT*)(const void*)a.aux)[gid]);
@@ -352,7 +355,7 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
                                           a.negate, nrm);
         }
     }
-    if (EPI == EPI_RESID && a.partial)
+    if ((EPI == EPI_RESID || EPI == EPI_RESN) && a.partial)
         block_sum_store(nrm, a.partial, blockIdx.x);
 }
 
@@ -646,6 +649,7 @@ int launch_general_stencil(const GeneralStencil& st, int dtype,
         case EPI_RESID: return launch_gen_t<double, EPI_RESID>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_UPDATE: return launch_gen_t<double, EPI_UPDATE>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         case EPI_POT: return launch_gen_t<double, EPI_POT>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
+        case EPI_RESN: return launch_gen_t<double, EPI_RESN>(st, n, bc, in, out, nin, ea, stream, nblocks_out);
         }
     } else if (dtype == 1) {
         switch (epi) {

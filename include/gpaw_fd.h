@@ -182,6 +182,16 @@ int gfd_batch_poisson_new(int G, int L, gfd_operator** ops,
                           const double* weights, gfd_batch_poisson** out);
 void gfd_batch_poisson_free(gfd_batch_poisson* b);
 
+/* Run each grid's levels L-1 .. nlev[g]-1 (its whole remaining hierarchy)
+ * in one LDS-resident workgroup per grid in place of the batched coarsest
+ * level.  ops / rst / itp: every grid's full hierarchy, grid after grid
+ * (nlev[g] operators and nlev[g] - 1 transformers each); weights[l]: the
+ * Jacobi weight of level l.  nlev == NULL switches it off. */
+int gfd_batch_poisson_set_coarse(gfd_batch_poisson* b, const int* nlev,
+                                 gfd_operator** ops, gfd_transformer** rst,
+                                 gfd_transformer** itp,
+                                 const double* weights);
+
 /* Points of all grids at one level (-1 for a bad level). */
 long gfd_batch_poisson_level_points(gfd_batch_poisson* b, int level);
 
@@ -269,6 +279,10 @@ int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
  * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
  * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
  * the tile of the tiled / full-row x-march kernel (value 0 clears);
+ * "coarse_lds" (0/1, default 1) runs the coarse levels of the Poisson
+ * V-cycles that fit in one workgroup's LDS as a single launch;
+ * "coarse_max_points" (default and 0: 4096) caps the points of its
+ * first level;
  * "xfer_passes" (0/1) runs K = 2 transfers as three 1-D passes instead of
  * the fused kernel.
  * Returns GFD_EINVAL for unknown keys or values. */

@@ -62,3 +62,37 @@ def test_mixed_precision_reaches_tolerance():
     a -= a.mean()
     b -= b.mean()
     assert relerr(a, b) < 1e-8
+
+
+@pytest.mark.parametrize('shape,pbc', [((48, 48, 48), True),
+                                       ((40, 24, 32), False),
+                                       ((64, 48, 32), (True, False, True))])
+def test_coarse_lds_matches_global(shape, pbc):
+    """The LDS-resident coarse levels against the per-level launches: the
+    same V-cycle up to rounding, FP64 and mixed precision."""
+    rng = np.random.default_rng(31)
+    gd = GridDescriptor(shape, np.array(shape) * 0.3, pbc)
+    rho = rand(rng, tuple(gd.n_c))
+    if gd.pbc_c.all():
+        rho -= rho.mean()
+    lib = _lib.load()
+    res = {}
+    try:
+        for on in (0, 1):
+            _lib.check(lib.gfd_set_tuning(b'coarse_lds', on), 'tuning')
+            s = _solver(gd, nn=3, eps=1e-24 * float((rho**2).sum()) * gd.dv,
+                        maxiter=300)
+            phi = dev(np.zeros(tuple(gd.n_c)))
+            say('coarse lds', on, shape, pbc)
+            n = s.solve_neutral(phi, dev(rho))
+            phi32 = dev(np.zeros(tuple(gd.n_c)))
+            m, rel = s.solve_mixed(phi32, dev(rho), 1e-10)
+            res[on] = (n, host(phi), m, rel, host(phi32))
+    finally:
+        lib.gfd_set_tuning(b'coarse_lds', 1)
+    (n0, p0, m0, r0, q0), (n1, p1, m1, r1, q1) = res[0], res[1]
+    assert abs(n1 - n0) <= 1
+    assert relerr(p1, p0) < 1e-10
+    assert r0 <= 1e-10 and r1 <= 1e-10 and abs(m1 - m0) <= 2
+    a, b = q1 - q1.mean(), q0 - q0.mean()
+    assert relerr(a, b) < 1e-8
