@@ -87,12 +87,14 @@ def traffic_from_profiles(name):
     return None
 
 
-def stage_poisson_192(dev, tol=1e-10, levels=5, inner=2, coarse=24):
+def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
     """Time to |L phi - rho| / |rho| <= tol from phi = 0 on the 192^3
-    problem: all-FP64 V-cycles (reference schedule, every level) against
-    FP32 V-cycles (2 pre / 2 post Jacobi sweeps, `levels` levels, `coarse`
-    sweeps on the coarsest, `inner` cycles per step) inside FP64 residual
-    refinement."""
+    problem: all-FP64 V-cycles (the reference schedule: V(2,1) with weight
+    2/3 on the fine level, V(4,4) below, every level) against FP32
+    V(2,2)-cycles (Jacobi weight w0 on the fine level, 1 below; the four
+    levels 192^3 .. 24^3 as grid-wide launches, 12^3 .. 3^3 in one
+    LDS-resident workgroup with `coarse` sweeps on 3^3; `inner` cycles per
+    step) inside FP64 residual refinement."""
     import torch
     from gpaw_amd import workloads
     from gpaw_amd.poisson import FDPoissonSolver
@@ -118,7 +120,7 @@ def stage_poisson_192(dev, tol=1e-10, levels=5, inner=2, coarse=24):
         t64.append(ms)
     rel64 = (s.last_error / (rn2 * gd.dv))**0.5
     phi = torch.zeros_like(drho)
-    kw = dict(levels=levels, pre=2, post=2, inner=inner, coarse_sweeps=coarse)
+    kw = dict(pre=2, post=2, inner=inner, coarse_sweeps=coarse, w0=w0)
     s.solve_mixed(phi, drho, tol, **kw)
     tmx = []
     for _ in range(3):
@@ -133,8 +135,9 @@ def stage_poisson_192(dev, tol=1e-10, levels=5, inner=2, coarse=24):
             'mixed_refinements': nmx, 'mixed_relres': relmx,
             'speedup_mixed_vs_fp64': round(min(t64) / min(tmx), 3),
             'mixed_err_vs_fp64': err,
-            'levels_fp64': s.levels + 1, 'levels_mixed': levels,
-            'inner_cycles': inner, 'coarse_sweeps': coarse}
+            'levels': s.levels + 1,
+            'mixed_cycle': 'V(2,2) FP32, w0=%g, %d per step, 4 grid-wide '
+                           'levels + LDS coarse solve' % (w0, inner)}
 
 
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
@@ -151,7 +154,7 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     phases = workloads.eigen_128_kpoints()
     vt = torch.from_numpy(workloads.eigen_128_potential(gd)).to(dev)
     kin = Laplace(gd, -0.5, 3, complex)
-    P = Preconditioner(gd, kin, complex, block=block)
+    P = Preconditioner(gd, kin, complex, block=block, fp32=True)
     g = torch.Generator(device=dev)
     g.manual_seed(1006)
     shape = (nbands, 128, 128, 128)
@@ -196,6 +199,18 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     # the overlap alone, for its MFMA rate
     from gpaw_amd.bands import overlap
     ms_s, _ = timed(lambda: overlap(psi, gd.dv, True, S), 3, 1)
+    # checks at full size: the rotated bands are orthonormal (their own
+    # overlap is the identity); the FP32 preconditioner against the FP64
+    # one on 8 bands
+    overlap(hpsi, gd.dv, False, S)
+    orth_err = float((S - torch.eye(nbands, dtype=S.dtype,
+                                    device=S.device)).abs().max())
+    Kpt.phase_cd = phases[-1]
+    P64 = Preconditioner(gd, kin, complex, block=8)
+    ref = P64(psi[:8], Kpt)
+    got = P(psi[:8], Kpt)
+    pre_err = float((got - ref).abs().max() / ref.abs().max())
+    del P64, ref, got
     X = 128**3
     nk = len(phases)
     herk_flop = nbands * (nbands + 1) / 2 * X * 8
@@ -210,8 +225,11 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
             'overlap_ms': round(ms_s, 3),
             'overlap_mfma_tflops': round(herk_flop / ms_s / 1e9, 2),
             'overlap_mfma_frac': round(herk_flop / ms_s / 1e9 / 78.6, 4),
+            'orthonormality_err': orth_err,
+            'precond_fp32_relerr_vs_fp64': pre_err,
             'bands': nbands, 'kpoints': nk, 'grid': [128, 128, 128],
-            'note': 'preconditioner applied to H psi; FP64 path'}
+            'note': 'FP32 preconditioner applied to H psi; H apply, '
+                    'overlap and Cholesky in FP64'}
 
 
 def cpu_baseline(gd, op_coefs, op_offsets, nbands_sample):

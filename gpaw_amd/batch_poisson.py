@@ -23,10 +23,13 @@ from .poisson import FDPoissonSolver, PoissonConvergenceError
 
 
 class BatchedPoissonSolver:
-    def __init__(self, nn=3, levels=4, coarse=True):
+    def __init__(self, nn=3, levels=4, coarse=True, w0=None):
+        """levels: hierarchy levels run as batched launches; w0: Jacobi
+        weight of the fine level (None: the reference's 2/3)."""
         self.nn = nn
         self.levels = levels
         self.coarse = coarse
+        self.w0 = w0
         self.solvers = []
         self._native = None
 
@@ -39,7 +42,9 @@ class BatchedPoissonSolver:
             s.set_grid_descriptor(gd)
             self.solvers.append(s)
         self.L = min([self.levels] + [s.levels + 1 for s in self.solvers])
-        self.weights = self.solvers[0].weights[:self.L]
+        self.weights = list(self.solvers[0].weights[:self.L])
+        if self.w0 is not None:
+            self.weights[0] = float(self.w0)
 
     def _handle(self):
         if self._native is None:
@@ -78,12 +83,13 @@ class BatchedPoissonSolver:
                for t in s.restrictors] or [None]
         itp = [t._handle().value for s in self.solvers
                for t in s.interpolators] or [None]
-        deep = max(self.solvers, key=lambda s: s.levels)
+        w = list(max(self.solvers, key=lambda s: s.levels).weights)
+        if self.w0 is not None:
+            w[0] = float(self.w0)
         _lib.check(lib.gfd_batch_poisson_set_coarse(
             self._native, (_lib.ctypes.c_int * len(nlev))(*nlev),
             (V * len(ops))(*ops), (V * len(rst))(*rst),
-            (V * len(itp))(*itp),
-            (_lib.ctypes.c_double * len(deep.weights))(*deep.weights)),
+            (V * len(itp))(*itp), (_lib.ctypes.c_double * len(w))(*w)),
             'set_coarse')
 
     def _free(self):

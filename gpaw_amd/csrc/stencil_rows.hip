@@ -4,6 +4,7 @@
 #include "rows.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 
 namespace gfd {
@@ -49,21 +50,29 @@ XmTile xr_default_tile(int R, int dtype, const int n[3], long nin)
     const int PZ = xr_pz(dtype);
     const int TZT = std::max(1, n[2] / PZ);
     // rows per thread: 2R+2 queue slots of PY x 16 bytes must stay well
-    // inside 256 VGPRs
+    // inside 256 VGPRs; one row per thread for 16-byte complex points
     const int pymax = R >= 3 ? 2 : 4;
-    // a segment count whose tile height divides n1: as many threads as
-    // possible up to 384 (measured best on 96^3 x 256 grids: 8 x 48
-    // threads, one row each, 0.77 ms against 0.81 for 6 x 48 and 0.92 for
-    // 6 x 48 with two rows), then one halo row per thread (s >= 2R), then
-    // fewer rows per thread
-    int best_score = -1;
+    const int pypref = xr_es(dtype) >= 16 ? 1 : 2;
+    // a segment count whose tile height divides n1, with 192-256 threads
+    // per block.  Measured (full-row sweeps): 256 x 96^3 f64 nn=3 best at
+    // 4 x 48 threads, 2 rows each, 24-32-plane x chunks (0.753 ms against
+    // 0.885 for 8 x 48 x 1); 128 x 128^3 c128 H apply best at 2 x 128
+    // threads, 1 row, 32-plane chunks (2.77 ms against 3.85 for 4 x 128).
+    // Small batches (a single Poisson level, below 32M points) keep up to
+    // 384 threads and one row per thread: with few tiles, occupancy wins
+    // over the halo rows saved (measured on the 192^3 V-cycles).
+    const bool small = (double)n[0] * n[1] * n[2] * (double)nin < 32.0e6;
+    int best_score = -1 << 30;
     for (int py = pymax; py >= 1; py >>= 1)
         for (int s = (2 * R + XR_HROWS - 1) / XR_HROWS; s * TZT <= XR_MAXT;
              s++) {
             if (n[1] % (s * py) != 0) continue;
             const int nt = s * TZT;
-            const int score = 2 * std::min(nt, 384) - (nt > 384 ? 64 : 0) +
-                              (s >= 2 * R ? 30 : 0) - 8 * (py - 1);
+            const int score =
+                small ? 2 * std::min(nt, 384) - (nt > 384 ? 64 : 0) +
+                            (s >= 2 * R ? 30 : 0) - 8 * (py - 1)
+                      : -std::abs(nt - 224) - (py == pypref ? 0 : 40) +
+                            (nt >= 128 ? 0 : -64);
             if (score > best_score) {
                 best_score = score;
                 t.nseg = s;
@@ -74,13 +83,15 @@ XmTile xr_default_tile(int R, int dtype, const int n[3], long nin)
         t.nseg = 1;  // nothing fits: xr_tile_ok rejects it
         return t;
     }
+    // x chunks of about 32 planes, more for small batches (~16 blocks per
+    // CU of 256), never below max(4R, 12) planes
     const long tiles = (long)(n[1] / (t.nseg * t.py)) * nin;
-    int chunks = 1;
-    // x chunks until ~16 blocks per CU (256 CUs)
-    if (tiles < 4096)
-        chunks = (int)std::max<long>(
-            1, std::min<long>((4096 + tiles - 1) / tiles, n[0] / (4 * R)));
-    t.xchunk = (n[0] + chunks - 1) / chunks;
+    long chunks = std::max<long>(1, (n[0] + 16) / 32);
+    if (tiles * chunks < 4096)
+        chunks = std::max<long>(chunks, (4096 + tiles - 1) / tiles);
+    chunks = std::max<long>(
+        1, std::min<long>(chunks, n[0] / std::max(4 * R, 12)));
+    t.xchunk = (int)((n[0] + chunks - 1) / chunks);
     return t;
 }
 
