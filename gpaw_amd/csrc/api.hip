@@ -1434,6 +1434,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         g_force_path = (int)value;
         return OK;
     }
+    if (std::strcmp(key, "xr_nt") == 0) {
+        xr_set_nt((int)value);
+        return OK;
+    }
     if (std::strcmp(key, "coarse_lds") == 0) {
         g_coarse_lds = value != 0;
         return OK;

@@ -37,7 +37,8 @@ struct BSt {
     long tile0;        // first tile of the grid in the launch
     int n[3];
     int periodic[3];
-    int tz, ty;        // tiles along z and y (tiles = n0 * ty * tz)
+    int tz, ty;        // tiles along z and y
+    int xc;            // x planes per tile (tiles = ceil(n0 / xc) ty tz)
     int has[3];
     double c0, w;
     double cp[3][BMAXR], cm[3][BMAXR];
@@ -78,11 +79,14 @@ __device__ inline bool bwrap(int& i, int n, int periodic)
 
 // T: data read, TO: data written (float output of an FP64 residual).
 // EPI_PLAIN, EPI_RELAX (aux = right-hand side) or EPI_RESID (aux = rhs,
-// partial[tile] = sum r^2).
+// partial[tile] = sum r^2).  A tile is BZ x PZ z points by BY rows of one
+// grid and marches over an x chunk: the centre values of planes x-R .. x+R
+// stay in a register queue, the z and y neighbours come through L1 / L2.
 template <typename T, typename TO, int R, int PZ, int EPI>
 __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
 {
     typedef typename ScalarOf<T>::type S;
+    constexpr int W = PZ + 2 * R;
     const int BZ = blockDim.x, BY = blockDim.y;
     for (long t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
         const BSt& d = a.g[bfind_tile(a.g, a.G, t)];
@@ -90,34 +94,66 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
         const int zt = (int)(lt % d.tz);
         lt /= d.tz;
         const int yt = (int)(lt % d.ty);
-        const int i0 = (int)(lt / d.ty);
+        const int xt = (int)(lt / d.ty);
         const int i1 = yt * BY + threadIdx.y;
         const int z0 = (zt * BZ + threadIdx.x) * PZ;
         const int n0 = d.n[0], n1 = d.n[1], n2 = d.n[2];
+        const int xs = xt * d.xc, xe = min(xs + d.xc, n0);
         double nrm = 0.0;
         if (i1 < n1 && z0 < n2) {
             const T* in = (const T*)a.in + d.off;
             const long plane = (long)n1 * n2;
-            const T* row = in + (long)i0 * plane + (long)i1 * n2;
-            T acc[PZ];
-            T ctr[PZ];
+            const long own = (long)i1 * n2;
+            // z window indices (wrapped; -1: zero or beyond the row)
+            int zi[W];
 #pragma unroll
-            for (int p = 0; p < PZ; p++) acc[p] = zero_of<T>();
-            // z: a window of PZ + 2R points of the own row
-            {
-                T wz[PZ + 2 * R];
-#pragma unroll
-                for (int k = 0; k < PZ + 2 * R; k++) {
-                    int j = z0 - R + k;
-                    const bool in_row = j >= 0 && j < n2;
-                    T v = zero_of<T>();
-                    if (in_row || bwrap(j, n2, d.periodic[2])) v = row[j];
-                    wz[k] = v;
+            for (int k = 0; k < W; k++) {
+                int j = z0 - R + k;
+                if (k >= R && k < R + PZ) {
+                    zi[k] = j < n2 ? j : -1;
+                } else {
+                    zi[k] = bwrap(j, n2, d.periodic[2]) ? j : -1;
                 }
+            }
+            // y neighbour rows (plane offsets; -1: zero boundary)
+            long yr[2][R];
 #pragma unroll
-                for (int p = 0; p < PZ; p++) {
-                    ctr[p] = wz[R + p];
-                    if (d.has[2]) {
+            for (int k = 1; k <= R; k++) {
+                int yp = i1 + k, ym = i1 - k;
+                yr[0][k - 1] = bwrap(yp, n1, d.periodic[1]) ? (long)yp * n2 : -1;
+                yr[1][k - 1] = bwrap(ym, n1, d.periodic[1]) ? (long)ym * n2 : -1;
+            }
+            // queue of centre chunks of planes x - R .. x + R
+            T q[2 * R + 1][PZ];
+            auto load_c = [&](int x, T* dst) {
+                int xx = x;
+                const bool ok = bwrap(xx, n0, d.periodic[0]);
+                const T* row = in + (long)xx * plane + own;
+#pragma unroll
+                for (int p = 0; p < PZ; p++)
+                    dst[p] = (ok && zi[R + p] >= 0) ? row[zi[R + p]]
+                                                    : zero_of<T>();
+            };
+#pragma unroll
+            for (int k = 0; k < 2 * R + 1; k++) load_c(xs - R + k, q[k]);
+            TO* out = (TO*)a.out;
+            for (int x = xs; x < xe; x++) {
+                const T* pl = in + (long)x * plane;
+                T acc[PZ];
+#pragma unroll
+                for (int p = 0; p < PZ; p++) acc[p] = zero_of<T>();
+                if (d.has[2]) {
+                    T wz[W];
+#pragma unroll
+                    for (int k = 0; k < W; k++) {
+                        if (k >= R && k < R + PZ)
+                            wz[k] = q[R][k - R];
+                        else
+                            wz[k] = zi[k] >= 0 ? pl[own + zi[k]]
+                                               : zero_of<T>();
+                    }
+#pragma unroll
+                    for (int p = 0; p < PZ; p++)
 #pragma unroll
                         for (int k = 1; k <= R; k++) {
                             acc[p] = fmac(acc[p], (S)d.cp[2][k - 1],
@@ -125,53 +161,58 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
                             acc[p] = fmac(acc[p], (S)d.cm[2][k - 1],
                                           wz[R + p - k]);
                         }
+                }
+                if (d.has[1]) {
+#pragma unroll
+                    for (int k = 1; k <= R; k++)
+#pragma unroll
+                        for (int sg = 0; sg < 2; sg++) {
+                            const long ro = yr[sg][k - 1];
+                            if (ro < 0) continue;
+                            const S c = sg ? (S)d.cm[1][k - 1]
+                                           : (S)d.cp[1][k - 1];
+#pragma unroll
+                            for (int p = 0; p < PZ; p++)
+                                if (zi[R + p] >= 0)
+                                    acc[p] = fmac(acc[p], c,
+                                                  pl[ro + zi[R + p]]);
+                        }
+                }
+                if (d.has[0]) {
+#pragma unroll
+                    for (int k = 1; k <= R; k++)
+#pragma unroll
+                        for (int p = 0; p < PZ; p++) {
+                            acc[p] = fmac(acc[p], (S)d.cp[0][k - 1],
+                                          q[R + k][p]);
+                            acc[p] = fmac(acc[p], (S)d.cm[0][k - 1],
+                                          q[R - k][p]);
+                        }
+                }
+                const long obase = d.off + (long)x * plane + own + z0;
+#pragma unroll
+                for (int p = 0; p < PZ; p++) {
+                    if (z0 + p >= n2) break;
+                    const T v = q[R][p];
+                    T r;
+                    if constexpr (EPI == EPI_PLAIN) {
+                        r = fmac(acc[p], (S)d.c0, v);
+                    } else if constexpr (EPI == EPI_RELAX) {
+                        const T f = ((const T*)a.aux)[obase + p];
+                        r = ((S)1.0 - (S)d.w) * v +
+                            (S)d.w * (f - acc[p]) / (S)d.c0;
+                    } else {
+                        const T f = ((const T*)a.aux)[obase + p];
+                        r = fmac(acc[p], (S)d.c0, v) - f;
+                        nrm += (double)r * (double)r;
                     }
+                    out[obase + p] = (TO)r;
                 }
-            }
-            // y and x: PZ points of the neighbouring rows
 #pragma unroll
-            for (int ax = 1; ax >= 0; ax--) {
-                if (!d.has[ax]) continue;
+                for (int k = 0; k < 2 * R; k++)
 #pragma unroll
-                for (int k = 1; k <= R; k++) {
-#pragma unroll
-                    for (int sgn = 0; sgn < 2; sgn++) {
-                        int ii = ax == 1 ? i1 + (sgn ? -k : k)
-                                         : i0 + (sgn ? -k : k);
-                        if (!bwrap(ii, ax == 1 ? n1 : n0, d.periodic[ax]))
-                            continue;
-                        const T* nb = ax == 1
-                                          ? in + (long)i0 * plane +
-                                                (long)ii * n2
-                                          : in + (long)ii * plane +
-                                                (long)i1 * n2;
-                        const S c = sgn ? (S)d.cm[ax][k - 1]
-                                        : (S)d.cp[ax][k - 1];
-#pragma unroll
-                        for (int p = 0; p < PZ; p++)
-                            if (z0 + p < n2)
-                                acc[p] = fmac(acc[p], c, nb[z0 + p]);
-                    }
-                }
-            }
-            const long obase = d.off + (long)i0 * plane + (long)i1 * n2 + z0;
-            TO* out = (TO*)a.out;
-#pragma unroll
-            for (int p = 0; p < PZ; p++) {
-                if (z0 + p >= n2) break;
-                T r;
-                if constexpr (EPI == EPI_PLAIN) {
-                    r = fmac(acc[p], (S)d.c0, ctr[p]);
-                } else if constexpr (EPI == EPI_RELAX) {
-                    const T f = ((const T*)a.aux)[obase + p];
-                    r = ((S)1.0 - (S)d.w) * ctr[p] +
-                        (S)d.w * (f - acc[p]) / (S)d.c0;
-                } else {
-                    const T f = ((const T*)a.aux)[obase + p];
-                    r = fmac(acc[p], (S)d.c0, ctr[p]) - f;
-                    nrm += (double)r * (double)r;
-                }
-                out[obase + p] = (TO)r;
+                    for (int p = 0; p < PZ; p++) q[k][p] = q[k + 1][p];
+                if (x + 1 < xe) load_c(x + R + 1, q[2 * R]);
             }
         }
         if (EPI == EPI_RESID && a.partial) block_sum_store(nrm, a.partial, t);
@@ -312,6 +353,7 @@ __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
 // resident blocks (0: one block per tile).
 struct BTile {
     int bz = 64, by = 4, pz = 1;
+    int xc = 16;                    // x planes per tile (0: whole grid)
     long blocks = 0;
 };
 
@@ -372,8 +414,9 @@ static void fill_tiles(BLevel& lv, const BTile& t)
     for (BSt& s : lv.st) {
         s.tz = (s.n[2] + t.bz * t.pz - 1) / (t.bz * t.pz);
         s.ty = (s.n[1] + t.by - 1) / t.by;
+        s.xc = t.xc > 0 ? std::min(t.xc, s.n[0]) : s.n[0];
         s.tile0 = t0;
-        t0 += (long)s.n[0] * s.ty * s.tz;
+        t0 += (long)((s.n[0] + s.xc - 1) / s.xc) * s.ty * s.tz;
     }
     lv.ntiles = t0;
 }
@@ -494,10 +537,11 @@ static int bnorms(const BLevel& lv, const double* partial, double* out,
 static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
                       const void* in, void* out, hipStream_t s)
 {
-    static const int shapes[][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16},
-                                    {64, 2}, {32, 4}};
+    static const int shapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 2},
+                                    {32, 4}, {16, 8}};
     static const int pzs[] = {1, 2, 4};
-    const long res[] = {0, (long)b->num_cu * 4, (long)b->num_cu * 8};
+    static const int xcs[] = {16, 32};
+    const long res[] = {0, (long)b->num_cu * 8};
     hipEvent_t e0, e1;
     GFD_CHECK_HIP(hipEventCreate(&e0));
     GFD_CHECK_HIP(hipEventCreate(&e1));
@@ -506,12 +550,13 @@ static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
     int st = OK;
     for (const auto& sh : shapes) {
         for (int pz : pzs) {
-            for (long nb : res) {
+            for (int ci = 0; ci < 4; ci++) {
                 BTile t;
                 t.bz = sh[0];
                 t.by = sh[1];
                 t.pz = pz;
-                t.blocks = nb;
+                t.xc = xcs[ci & 1];
+                t.blocks = res[ci >> 1];
                 fill_tiles(lv, t);
                 if ((st = upload_level(lv, s)) != OK) break;
                 const int ep = EPI_PLAIN;
@@ -531,6 +576,7 @@ static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
                     best = ms;
                     bt = t;
                 }
+                if (st != OK) break;
             }
             if (st != OK) break;
         }

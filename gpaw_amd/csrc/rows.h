@@ -10,6 +10,10 @@ namespace gfd {
 constexpr int XR_MAXT = 512;
 constexpr int XR_HROWS = 3;  // halo rows loaded per thread per plane
 
+// Non-temporal hints of the full-row kernel for arrays of `bytes`: bit 0
+// output stores, bit 1 own-row loads (xr_set_nt in stencil.h).
+int xr_nt_flags(long bytes);
+
 // Launch k_xrow for an already validated tile; ENOTSUP_ for epilogues the
 // data type does not instantiate.
 int row_launch_f64(const AxisStencil& st, const XmTile& tl, const int n[3],

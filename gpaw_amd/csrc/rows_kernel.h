@@ -50,6 +50,7 @@ struct XrArgs {
     BC bc;
     double* partial;
     const double* pot;
+    int nt;              // non-temporal hints (xr_nt_flags)
 };
 
 
@@ -60,6 +61,27 @@ __device__ inline Chunk<T, PZ> zero_chunk()
 #pragma unroll
     for (int i = 0; i < PZ; i++) c.v[i] = zero_of<T>();
     return c;
+}
+
+// 16-byte loads / stores with the non-temporal hint (streamed data that
+// no other block reads again)
+typedef double nt16 __attribute__((ext_vector_type(2)));
+template <typename C>
+__device__ inline C ld16nt(const C* p)
+{
+    static_assert(sizeof(C) == 16, "16-byte chunks only");
+    const nt16 v = __builtin_nontemporal_load(reinterpret_cast<const nt16*>(p));
+    C c;
+    __builtin_memcpy(&c, &v, 16);
+    return c;
+}
+template <typename C>
+__device__ inline void st16nt(const C& c, C* p)
+{
+    static_assert(sizeof(C) == 16, "16-byte chunks only");
+    nt16 v;
+    __builtin_memcpy(&v, &c, 16);
+    __builtin_nontemporal_store(v, reinterpret_cast<nt16*>(p));
 }
 
 template <typename T, int PZ>
@@ -178,7 +200,10 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
         const T* p = gin + base + own0;
 #pragma unroll
         for (int j = 0; j < PY; j++) {
-            dst[j] = *reinterpret_cast<const C*>(p + j * n2);
+            if (a.nt & 2)
+                dst[j] = ld16nt(reinterpret_cast<const C*>(p + j * n2));
+            else
+                dst[j] = *reinterpret_cast<const C*>(p + j * n2);
             if constexpr (IsCplx<T>::value)
                 if (phx) dst[j] = phase_chunk<T, PZ>(dst[j], fx);
         }
@@ -343,7 +368,10 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                                                        av.v[p], a.c0, a.w,
                                                        a.s, a.negate, nrm);
                     }
-                    *reinterpret_cast<C*>(a.out + idx) = r;
+                    if (a.nt & 1)
+                        st16nt(r, reinterpret_cast<C*>(a.out + idx));
+                    else
+                        *reinterpret_cast<C*>(a.out + idx) = r;
                 }
             }
         }
@@ -395,6 +423,7 @@ static int launch_xr_t(const AxisStencil& st, const XmTile& tl,
     a.bc = bc;
     a.partial = ea.partial;
     a.pot = ea.pot;
+    a.nt = xr_nt_flags((long)n[0] * n[1] * n[2] * nin * (long)sizeof(T));
     const int TZT = n[2] / PZ;
     const size_t shm =
         (st.has[1] || st.has[2]) ? 2 * (size_t)(TY + 2 * R) * n[2] * sizeof(T)

@@ -80,6 +80,9 @@ bool row_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
 long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
                             long nin);
 XmTile xr_default_tile(int R, int dtype, const int n[3], long nin);
+// Non-temporal hints of the full-row kernel: -1 automatic, else bit 0
+// output stores, bit 1 own-row loads.
+void xr_set_nt(int flags);
 void xr_set_override(const XmTile* t);
 
 // Whether the x-marching kernel takes this stencil / grid (else general).

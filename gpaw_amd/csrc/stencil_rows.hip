@@ -25,6 +25,19 @@ static bool xr_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
            t.xchunk >= 1 && lds <= 150 * 1024;
 }
 
+static int g_xr_nt = -1;
+
+void xr_set_nt(int flags) { g_xr_nt = flags; }
+
+int xr_nt_flags(long bytes)
+{
+    if (g_xr_nt >= 0) return g_xr_nt;
+    // non-temporal output stores for arrays far beyond the 256 MB
+    // Infinity Cache (1-2 % on 256 x 96^3 f64); non-temporal loads were
+    // slower in every measured case
+    return bytes > (512L << 20) ? 1 : 0;
+}
+
 static std::mutex g_xr_mu;
 static bool g_xr_override = false;
 static XmTile g_xr_tile;
@@ -83,10 +96,12 @@ XmTile xr_default_tile(int R, int dtype, const int n[3], long nin)
         t.nseg = 1;  // nothing fits: xr_tile_ok rejects it
         return t;
     }
-    // x chunks of about 32 planes, more for small batches (~16 blocks per
-    // CU of 256), never below max(4R, 12) planes
+    // x chunks of 16-32 planes, more for small batches (~16 blocks per CU
+    // of 256), never below max(4R, 12) planes
     const long tiles = (long)(n[1] / (t.nseg * t.py)) * nin;
-    long chunks = std::max<long>(1, (n[0] + 16) / 32);
+    // (measured: 16-plane chunks best for 8-byte data, 32 for c128)
+    const int xtarget = xr_es(dtype) >= 16 ? 32 : 16;
+    long chunks = std::max<long>(1, (n[0] + xtarget / 2) / xtarget);
     if (tiles * chunks < 4096)
         chunks = std::max<long>(chunks, (4096 + tiles - 1) / tiles);
     chunks = std::max<long>(
