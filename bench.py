@@ -10,9 +10,9 @@ restrict / interpolate nn=3 on 4 densities 96^3 <-> 48^3, the FP64 multigrid
 Poisson solve of the 48^3 four-Gaussian problem (eps 2e-10), the kinetic
 preconditioner on 32 complex bands, the 192^3 Poisson time to a 1e-10
 relative residual (all-FP64 V-cycles against FP32 V-cycles inside FP64
-refinement) and the FD eigensolver step on 1024 complex bands of 128^3 at
-4 k-points (H apply, preconditioner, FP64 MFMA overlap, Cholesky
-orthonormalisation).
+refinement), the 48 uneven Poisson problems solved as one batch, and the FD
+eigensolver step on 1024 complex bands of 128^3 at 4 k-points (H apply,
+preconditioner, FP64 MFMA overlap, Cholesky orthonormalisation).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--no-stages] [--no-eigen]
@@ -53,6 +53,9 @@ def parse():
                    help='counter runs: 3 headline applies and 3 copies of '
                         'the same 3.62 GB (calibration), no timing line')
     p.add_argument('--eigen-bands', type=int, default=1024)
+    p.add_argument('--stage', choices=['poisson192', 'uneven48', 'eigen'],
+                   help='run only this stage and print its result (for '
+                        'profiling)')
     return p.parse_args()
 
 
@@ -138,6 +141,73 @@ def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
             'levels': s.levels + 1,
             'mixed_cycle': 'V(2,2) FP32, w0=%g, %d per step, 4 grid-wide '
                            'levels + LDS coarse solve' % (w0, inner)}
+
+
+def stage_uneven_48(dev, tol=1e-10):
+    """Config 4: 48 independent Poisson problems of uneven shape (sides
+    from {40, .., 160}, alternately periodic / zero boundary), solved
+    together (FP32 V(2,2) in batched persistent launches + per-grid LDS
+    coarse levels, FP64 refinement, one graph per step) against the 48
+    FP64 reference-schedule solves one after the other."""
+    import torch
+    from gpaw_amd import workloads
+    from gpaw_amd.batch_poisson import BatchedPoissonSolver
+    from gpaw_amd.poisson import FDPoissonSolver
+    problems = workloads.uneven_48()
+    gds = [g for g, _ in problems]
+    rhos = [torch.from_numpy(r).to(dev) for _, r in problems]
+    npts = sum(int(np.prod(g.n_c)) for g in gds)
+    b = BatchedPoissonSolver(nn=3, levels=4, w0=0.8)
+    b.set_grid_descriptors(gds)
+    t0 = time.perf_counter()
+    b.tune()
+    tune_s = time.perf_counter() - t0
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3, r
+
+    phis = [torch.zeros_like(r) for r in rhos]
+    b.solve_mixed(phis, rhos, tol, coarse_sweeps=64)
+    best = None
+    for _ in range(2):
+        phis = [torch.zeros_like(r) for r in rhos]
+        ms, (n, rel) = wall(lambda: b.solve_mixed(phis, rhos, tol,
+                                                  coarse_sweeps=64))
+        best = ms if best is None else min(best, ms)
+    solvers = []
+    for g, r in zip(gds, rhos):
+        s = FDPoissonSolver(nn=3, eps=tol**2 * float((r**2).sum()) * g.dv,
+                            maxiter=1000)
+        s.set_grid_descriptor(g)
+        solvers.append(s)
+    refs = [torch.zeros_like(r) for r in rhos]
+
+    def run64():
+        its = []
+        for s, r, p in zip(solvers, rhos, refs):
+            p.zero_()
+            its.append(s.solve_neutral(p, r))
+        return its
+
+    run64()
+    ms64, its = wall(run64)
+    err = 0.0
+    for g, p, q in zip(gds, phis, refs):
+        a, c = p, q
+        if g.pbc_c.all():
+            a, c = a - a.mean(), c - c.mean()
+        err = max(err, float((a - c).abs().max() / c.abs().max()))
+    return {'batched_ms': round(best, 2), 'refinements': n,
+            'max_relres': float(rel.max()), 'tune_s': round(tune_s, 2),
+            'fp64_sequential_ms': round(ms64, 2),
+            'fp64_vcycles_total': int(sum(its)),
+            'speedup': round(ms64 / best, 2), 'max_err_vs_fp64': err,
+            'problems': len(gds), 'points': npts,
+            'tiles': {str(k): v for k, v in b.tiles().items()}}
 
 
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
@@ -289,6 +359,14 @@ def main():
     _lib.load()
     torch.cuda.init()
     dev = torch.device('cuda', 0)
+    if args.stage:
+        fn = {'poisson192': lambda: stage_poisson_192(dev),
+              'uneven48': lambda: stage_uneven_48(dev),
+              'eigen': lambda: stage_eigen_128(dev, args.eigen_bands,
+                                               steps=1, warmup=1)}
+        print(json.dumps({args.stage: fn[args.stage]()}, default=float),
+              flush=True)
+        return
 
     gd = workloads.bands_96_gd()
     nb = args.bands
@@ -412,6 +490,9 @@ def main():
         stages['poisson_192_time_to_1e-10'] = stage_poisson_192(dev)
         log('[bench] poisson 192', stages['poisson_192_time_to_1e-10'])
         del res, out, P
+        torch.cuda.empty_cache()
+        stages['uneven_48_batched_poisson'] = stage_uneven_48(dev)
+        log('[bench] uneven 48', stages['uneven_48_batched_poisson'])
         torch.cuda.empty_cache()
         if not args.no_eigen:
             stages['eigen_step_c128_128'] = stage_eigen_128(

@@ -71,23 +71,37 @@ __global__ void __launch_bounds__(512) k_gram(GramArgs a)
             acci[r][c] = d4{0.0, 0.0, 0.0, 0.0};
         }
 
-    // stage loader: 128 rows x 16 points of A and of B, 4 per thread
+    // stage loader: 128 rows x 16 points of A and of B, 4 per thread; the
+    // next stage's values are fetched into registers while the current
+    // stage is multiplied out of LDS
     const int lrow = t >> 2, lkq = (t & 3) * 4;
-    for (long ks = kb; ks < ke; ks += GK) {
+    const bool arow = i0 + lrow < a.M, brow = j0 + lrow < a.N;
+    const double2* pa = a.A + (long)(i0 + lrow) * a.lda;
+    const double2* pb = a.B + (long)(j0 + lrow) * a.ldb;
+    double2 va[4], vb[4];
+    auto fetch = [&](long ks) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const long k = ks + lkq + q;
-            double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+            va[q] = make_double2(0.0, 0.0);
+            vb[q] = make_double2(0.0, 0.0);
             if (k < ke) {
-                if (i0 + lrow < a.M) va = a.A[(long)(i0 + lrow) * a.lda + k];
-                if (j0 + lrow < a.N) vb = a.B[(long)(j0 + lrow) * a.ldb + k];
+                if (arow) va[q] = pa[k];
+                if (brow) vb[q] = pb[k];
             }
-            sAr[lkq + q][lrow] = va.x;
-            sAi[lkq + q][lrow] = va.y;
-            sBr[lkq + q][lrow] = vb.x;
-            sBi[lkq + q][lrow] = vb.y;
+        }
+    };
+    fetch(kb);
+    for (long ks = kb; ks < ke; ks += GK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sAr[lkq + q][lrow] = va[q].x;
+            sAi[lkq + q][lrow] = va[q].y;
+            sBr[lkq + q][lrow] = vb[q].x;
+            sBi[lkq + q][lrow] = vb[q].y;
         }
         __syncthreads();
+        if (ks + GK < ke) fetch(ks + GK);
 #pragma unroll
         for (int kk = 0; kk < GK / 4; kk++) {
             const int kl = kk * 4 + lk;

@@ -9,6 +9,7 @@ namespace gfd {
 
 constexpr int XR_MAXT = 512;
 constexpr int XR_HROWS = 3;  // halo rows loaded per thread per plane
+constexpr int XR_LOOK = 1;   // own planes loaded ahead of the stencil reach
 
 // Non-temporal hints of the full-row kernel for arrays of `bytes`: bit 0
 // output stores, bit 1 own-row loads (xr_set_nt in stencil.h).

@@ -102,7 +102,9 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
     constexpr int RC = (R + PZ - 1) / PZ;  // z-window chunks on each side
     constexpr int NW = 2 * RC + 1;         // z-window chunks
     constexpr int RP = RC * PZ;
-    constexpr int U = 2 * R + 2;           // queue slots = unroll length
+    // queue slots = unroll length: planes x-R .. x+R plus XR_LOOK planes
+    // in flight
+    constexpr int U = 2 * R + 1 + XR_LOOK;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
 
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
     C q[U][PY];
     C hv[XR_HROWS];
 #pragma unroll
-    for (int k = -R; k <= R; k++) load_own(xs + k, q[(k + U) % U]);
+    for (int k = -R; k < R + XR_LOOK; k++) load_own(xs + k, q[(k + U) % U]);
     if (anyhalo) load_halo(xs, hv);
 
     double nrm = 0.0;
@@ -243,7 +245,8 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
         for (int u = 0; u < U; u++) {
             const int x = xg + u;
             if (x < xe) {
-                T* buf = lds + (u & 1) * BUF;
+                // alternate LDS planes by step parity (U may be odd)
+                T* buf = lds + ((x - xs) & 1) * BUF;
                 if (a.hasy || a.hasz) {
 #pragma unroll
                     for (int j = 0; j < PY; j++)
@@ -256,10 +259,9 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                 }
                 // halo first: its wait at the next step leaves the own-plane
                 // loads issued after it in flight
-                if (x + 1 < xe) {
-                    if (anyhalo) load_halo(x + 1, hv);
-                    load_own(x + R + 1, q[(u + R + 1) % U]);
-                }
+                if (x + 1 < xe && anyhalo) load_halo(x + 1, hv);
+                if (x + XR_LOOK < xe)
+                    load_own(x + R + XR_LOOK, q[(u + R + XR_LOOK) % U]);
                 if (a.hasy || a.hasz) __syncthreads();
 
                 const long obase = b * a.ng + (long)x * plane + own0;

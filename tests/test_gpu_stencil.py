@@ -149,3 +149,43 @@ def test_residual_norm():
             n2 = op.residual(d, dev(rho), r, norm=True)
             assert relerr(host(r), r_ref) < TOL
             assert n2 == pytest.approx(float((r_ref**2).sum()), rel=1e-12)
+
+
+@pytest.mark.parametrize('nt', [1, 3])
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_row_kernel_nontemporal(nt, dtype):
+    """The non-temporal load/store variants of the full-row kernel (stores
+    are on by default above 512 MB) at test sizes, forced on."""
+    from gpaw_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(60 + nt)
+    gd = GridDescriptor((24, 32, 16), (3.0, 4.0, 2.0), (True, False, True))
+    op = Laplace(gd, -0.5, 3, dtype)
+    a = rand(rng, (5,) + tuple(gd.n_c), dtype)
+    ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+    ref = ref_apply(op, a, ph)
+    try:
+        _lib.check(lib.gfd_set_tuning(b'xr_nt', nt), 'tuning')
+        with kernel_path('row'):
+            say('row kernel nt', nt, dtype.__name__)
+            out = gpu_apply(op, a, ph)
+    finally:
+        lib.gfd_set_tuning(b'xr_nt', -1)
+    assert relerr(out, ref) < TOL
+
+
+def test_row_kernel_default_tiles_large_batch():
+    """A batch big enough for the large-batch tile choice (several x chunks
+    per grid, two rows per thread) against the oracle on two bands."""
+    rng = np.random.default_rng(61)
+    gd = GridDescriptor((48, 48, 48), (9.0, 9.0, 9.0))
+    op = Laplace(gd, 1.0, 3)
+    nb = 300   # 33M points
+    import torch
+    d = torch.rand((nb, 48, 48, 48), dtype=torch.float64, device='cuda')
+    out = torch.empty_like(d)
+    say('large batch row tiles')
+    op.apply(d, out)
+    for band in (0, nb - 1):
+        ref = ref_apply(op, host(d[band]))
+        assert relerr(host(out[band]), ref) < TOL
