@@ -1049,12 +1049,29 @@ static int ensure_stream(gfd_poisson* p)
 
 // Capture fn(stream) into an executable graph.  Buffers that the captured
 // work may grow are sized first: nothing allocates during a capture.
+// One eager PLAIN pass per level on the scratch arrays lets the stencil
+// tile tuner time its candidates before a graph freezes the tiles (the
+// tuner never runs under capture); tmp and res are rewritten before they
+// are read in every V-cycle.
+static int pretune_levels(gfd_poisson* p, Levels& lv, hipStream_t s)
+{
+    for (int l = 0; l <= lv.L && l <= p->L; l++) {
+        EpiArgs e{};
+        e.c0 = centre_coef(p->ops[l]);
+        GFD_TRY(op_pass(p->ops[l], lv.dtype, lv.tmp[l], lv.res[l], 1,
+                        nullptr, EPI_PLAIN, e, s, nullptr));
+    }
+    return OK;
+}
+
 template <typename F>
 static int capture(gfd_poisson* p, hipGraphExec_t* exec, F fn)
 {
     gfd_operator* op0 = p->ops[0];
     GFD_TRY(op0->partial.ensure(
         sizeof(double) * (op_max_blocks(op0, 0, 1) + 1), p->gs));
+    GFD_TRY(pretune_levels(p, p->lv, p->gs));
+    if (p->lv32.L >= 0) GFD_TRY(pretune_levels(p, p->lv32, p->gs));
     hipGraph_t g = nullptr;
     GFD_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
     int st = fn(p->gs);
@@ -1432,6 +1449,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
             return EINVAL_;
         }
         g_force_path = (int)value;
+        return OK;
+    }
+    if (std::strcmp(key, "autotune") == 0) {
+        xr_set_autotune(value != 0);
         return OK;
     }
     if (std::strcmp(key, "xr_nt") == 0) {

@@ -80,6 +80,10 @@ bool row_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
 long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
                             long nin);
 XmTile xr_default_tile(int R, int dtype, const int n[3], long nin);
+// Timing autotuner of the full-row kernel's tile (on by default; off also
+// forgets the tiles found).
+void xr_set_autotune(bool on);
+
 // Non-temporal hints of the full-row kernel: -1 automatic, else bit 0
 // output stores, bit 1 own-row loads.
 void xr_set_nt(int flags);

@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
 #include <mutex>
+#include <vector>
 
 namespace gfd {
 
@@ -117,25 +119,54 @@ bool row_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
     return xr_tile_ok(xr_default_tile(st.R, dtype, n, nin), st.R, dtype, n);
 }
 
+// Smallest x chunk any tile (heuristic or tuned) uses: the block count
+// bound below must hold for every tile launch_row_stencil may pick.
+constexpr int XR_XCMIN = 4;
+
 long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
                             long nin)
 {
-    const XmTile t = xr_default_tile(st.R, dtype, n, nin);
-    const int TY = t.nseg * t.py;
-    return (long)((n[1] + TY - 1) / TY) * ((n[0] + t.xchunk - 1) / t.xchunk) *
-           nin;
+    (void)st;
+    (void)dtype;
+    return (long)n[1] * ((n[0] + XR_XCMIN - 1) / XR_XCMIN) * nin;
 }
 
-int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
-                       const BC& bc, const void* in, void* out, long nin,
-                       int epi, const EpiArgs& ea, hipStream_t stream,
-                       int* nblocks_out)
-{
-    const XmTile tl = xr_default_tile(st.R, dtype, n, nin);
-    if (!xr_tile_ok(tl, st.R, dtype, n)) {
-        set_error("x-row tile does not fit this grid");
-        return ENOTSUP_;
+// ---------------------------------------------------------------- tuner
+// The first launch of a (radius, type, shape, batch) times candidate tiles
+// with a PLAIN pass from the call's input into its output (overwritten by
+// the real pass right after) and keeps the fastest.  Not while the stream
+// is being captured, nor when the output aliases an input.
+
+struct XrKey {
+    int R, dtype, n0, n1, n2;
+    long nin;
+    bool operator<(const XrKey& o) const
+    {
+        if (R != o.R) return R < o.R;
+        if (dtype != o.dtype) return dtype < o.dtype;
+        if (n0 != o.n0) return n0 < o.n0;
+        if (n1 != o.n1) return n1 < o.n1;
+        if (n2 != o.n2) return n2 < o.n2;
+        return nin < o.nin;
     }
+};
+
+static std::mutex g_tune_mu;
+static std::map<XrKey, XmTile> g_tuned;
+static bool g_autotune = true;
+
+void xr_set_autotune(bool on)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_autotune = on;
+    if (!on) g_tuned.clear();
+}
+
+static int row_launch(const AxisStencil& st, const XmTile& tl, int dtype,
+                      const int n[3], const BC& bc, const void* in, void* out,
+                      long nin, int epi, const EpiArgs& ea, hipStream_t stream,
+                      int* nblocks_out)
+{
     switch (dtype) {
     case 0: return row_launch_f64(st, tl, n, bc, in, out, nin, epi, ea, stream, nblocks_out);
     case 1: return row_launch_c128(st, tl, n, bc, in, out, nin, epi, ea, stream, nblocks_out);
@@ -144,6 +175,129 @@ int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
     }
     set_error("x-row: unsupported dtype %d", dtype);
     return ENOTSUP_;
+}
+
+static XmTile xr_tune(const AxisStencil& st, int dtype, const int n[3],
+                      const BC& bc, const void* in, void* out, long nin,
+                      hipStream_t stream, const XmTile& heur)
+{
+    const int R = st.R;
+    const int PZ = xr_pz(dtype);
+    const int TZT = std::max(1, n[2] / PZ);
+    // time on at most ~32M points of the batch (the ranking of tiles
+    // does not change beyond a few waves per CU)
+    const long per = (long)n[0] * n[1] * n[2];
+    const long nt_in = std::max(1L, std::min(nin, (32L << 20) / per));
+    std::vector<XmTile> cand;
+    cand.push_back(heur);
+    const int pymax = R >= 3 ? 2 : 4;
+    static const int xcs[] = {4, 8, 12, 16, 24, 32, 48, 64};
+    for (int py = 1; py <= pymax; py <<= 1)
+        for (int sg = (2 * R + XR_HROWS - 1) / XR_HROWS; sg * TZT <= XR_MAXT;
+             sg++) {
+            if (n[1] % (sg * py) != 0) continue;
+            if (sg * TZT < 64 && sg * TZT < n[2] / PZ * (n[1] / py)) {
+                // fewer than a wave of threads while more rows exist
+                continue;
+            }
+            for (int xc : xcs) {
+                if (xc < std::max(2 * R, XR_XCMIN) || xc > n[0]) continue;
+                XmTile t;
+                t.tz = n[2];
+                t.nseg = sg;
+                t.py = py;
+                t.xchunk = xc;
+                if (xr_tile_ok(t, R, dtype, n)) cand.push_back(t);
+            }
+            XmTile t;
+            t.tz = n[2];
+            t.nseg = sg;
+            t.py = py;
+            t.xchunk = n[0];
+            if (xr_tile_ok(t, R, dtype, n)) cand.push_back(t);
+        }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return heur;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return heur;
+    }
+    EpiArgs ea{};
+    ea.c0 = st.c0;
+    XmTile best = heur;
+    float best_ms = 1e30f;
+    for (const XmTile& t : cand) {
+        // one untimed pass, then two timed
+        if (row_launch(st, t, dtype, n, bc, in, out, nt_in, EPI_PLAIN, ea,
+                       stream, nullptr) != OK)
+            continue;
+        (void)hipEventRecord(e0, stream);
+        int st2 = OK;
+        for (int r = 0; r < 2 && st2 == OK; r++)
+            st2 = row_launch(st, t, dtype, n, bc, in, out, nt_in, EPI_PLAIN,
+                             ea, stream, nullptr);
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess || st2 != OK) {
+            (void)hipGetLastError();
+            continue;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_error("");
+    return best;
+}
+
+static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
+                           const BC& bc, const void* in, void* out, long nin,
+                           int epi, const EpiArgs& ea, hipStream_t stream)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_xr_mu);
+        if (g_xr_override) return g_xr_tile;
+    }
+    const XmTile heur = xr_default_tile(st.R, dtype, n, nin);
+    const XrKey key{st.R, dtype, n[0], n[1], n[2], nin};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (!g_autotune) return heur;
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return heur;
+    }
+    if (cs != hipStreamCaptureStatusNone || in == out ||
+        (ea.aux && ea.aux == out) || (ea.pot && (const void*)ea.pot == out))
+        return heur;
+    (void)epi;
+    const XmTile t = xr_tune(st, dtype, n, bc, in, out, nin, stream, heur);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = t;
+    return t;
+}
+
+int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
+                       const BC& bc, const void* in, void* out, long nin,
+                       int epi, const EpiArgs& ea, hipStream_t stream,
+                       int* nblocks_out)
+{
+    const XmTile tl =
+        xr_pick_tile(st, dtype, n, bc, in, out, nin, epi, ea, stream);
+    if (!xr_tile_ok(tl, st.R, dtype, n)) {
+        set_error("x-row tile does not fit this grid");
+        return ENOTSUP_;
+    }
+    return row_launch(st, tl, dtype, n, bc, in, out, nin, epi, ea, stream,
+                      nblocks_out);
 }
 
 }  // namespace gfd

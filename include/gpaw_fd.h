@@ -279,6 +279,9 @@ int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
  * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
  * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
  * the tile of the tiled / full-row x-march kernel (value 0 clears);
+ * "autotune" (0/1, default 1) times candidate full-row tiles on the first
+ * launch of each shape (off: heuristic tiles); "xr_nt" (-1 automatic, else
+ * bit 0 non-temporal stores, bit 1 non-temporal loads);
  * "coarse_lds" (0/1, default 1) runs the coarse levels of the Poisson
  * V-cycles that fit in one workgroup's LDS as a single launch;
  * "coarse_max_points" (default and 0: 4096) caps the points of its
