@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
     constexpr int W = PZ + 2 * R;
     const int BZ = blockDim.x, BY = blockDim.y;
     for (long t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const BSt& d = a.g[bfind_tile(a.g, a.G, t)];
+        // by value: stores to out could alias a reference into the table,
+        // forcing a reload of every field after each store
+        const BSt d = a.g[bfind_tile(a.g, a.G, t)];
         long lt = t - d.tile0;
         const int zt = (int)(lt % d.tz);
         lt /= d.tz;
@@ -134,11 +136,39 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
                     dst[p] = (ok && zi[R + p] >= 0) ? row[zi[R + p]]
                                                     : zero_of<T>();
             };
+            // z-window ends and y rows of plane x (loaded one plane ahead so
+            // their latency overlaps the previous plane's arithmetic)
+            T zx[2 * R], yv[2 * R][PZ];
+            auto load_side = [&](int x, T* zo, T (*yo)[PZ]) {
+                const T* pl = in + (long)x * plane;
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    zo[k] = (d.has[2] && zi[k] >= 0) ? pl[own + zi[k]]
+                                                     : zero_of<T>();
+                    zo[R + k] = (d.has[2] && zi[R + PZ + k] >= 0)
+                                    ? pl[own + zi[R + PZ + k]]
+                                    : zero_of<T>();
+                }
+#pragma unroll
+                for (int k = 0; k < R; k++)
+#pragma unroll
+                    for (int sg = 0; sg < 2; sg++) {
+                        const long ro = yr[sg][k];
+#pragma unroll
+                        for (int p = 0; p < PZ; p++)
+                            yo[2 * k + sg][p] =
+                                (d.has[1] && ro >= 0 && zi[R + p] >= 0)
+                                    ? pl[ro + zi[R + p]]
+                                    : zero_of<T>();
+                    }
+            };
 #pragma unroll
             for (int k = 0; k < 2 * R + 1; k++) load_c(xs - R + k, q[k]);
+            load_side(xs, zx, yv);
             TO* out = (TO*)a.out;
             for (int x = xs; x < xe; x++) {
-                const T* pl = in + (long)x * plane;
+                T nzx[2 * R], nyv[2 * R][PZ];
+                if (x + 1 < xe) load_side(x + 1, nzx, nyv);
                 T acc[PZ];
 #pragma unroll
                 for (int p = 0; p < PZ; p++) acc[p] = zero_of<T>();
@@ -146,11 +176,12 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
                     T wz[W];
 #pragma unroll
                     for (int k = 0; k < W; k++) {
-                        if (k >= R && k < R + PZ)
+                        if (k < R)
+                            wz[k] = zx[k];
+                        else if (k < R + PZ)
                             wz[k] = q[R][k - R];
                         else
-                            wz[k] = zi[k] >= 0 ? pl[own + zi[k]]
-                                               : zero_of<T>();
+                            wz[k] = zx[R + k - R - PZ];
                     }
 #pragma unroll
                     for (int p = 0; p < PZ; p++)
@@ -166,16 +197,11 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
 #pragma unroll
                     for (int k = 1; k <= R; k++)
 #pragma unroll
-                        for (int sg = 0; sg < 2; sg++) {
-                            const long ro = yr[sg][k - 1];
-                            if (ro < 0) continue;
-                            const S c = sg ? (S)d.cm[1][k - 1]
-                                           : (S)d.cp[1][k - 1];
-#pragma unroll
-                            for (int p = 0; p < PZ; p++)
-                                if (zi[R + p] >= 0)
-                                    acc[p] = fmac(acc[p], c,
-                                                  pl[ro + zi[R + p]]);
+                        for (int p = 0; p < PZ; p++) {
+                            acc[p] = fmac(acc[p], (S)d.cp[1][k - 1],
+                                          yv[2 * (k - 1)][p]);
+                            acc[p] = fmac(acc[p], (S)d.cm[1][k - 1],
+                                          yv[2 * (k - 1) + 1][p]);
                         }
                 }
                 if (d.has[0]) {
@@ -212,7 +238,15 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
                 for (int k = 0; k < 2 * R; k++)
 #pragma unroll
                     for (int p = 0; p < PZ; p++) q[k][p] = q[k + 1][p];
-                if (x + 1 < xe) load_c(x + R + 1, q[2 * R]);
+                if (x + 1 < xe) {
+                    load_c(x + R + 1, q[2 * R]);
+#pragma unroll
+                    for (int k = 0; k < 2 * R; k++) {
+                        zx[k] = nzx[k];
+#pragma unroll
+                        for (int p = 0; p < PZ; p++) yv[k][p] = nyv[k][p];
+                    }
+                }
             }
         }
         if (EPI == EPI_RESID && a.partial) block_sum_store(nrm, a.partial, t);
@@ -288,9 +322,15 @@ template <typename T, bool INTERP>
 __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
 {
     typedef typename ScalarOf<T>::type S;
-    for (long gid = (long)blockIdx.x * 256 + threadIdx.x; gid < a.total;
-         gid += (long)gridDim.x * 256) {
-        const BXf& d = a.g[bfind_point(a.g, a.G, gid)];
+    for (long b0 = (long)blockIdx.x * 256; b0 < a.total;
+         b0 += (long)gridDim.x * 256) {
+        const long gid = b0 + threadIdx.x;
+        if (gid >= a.total) break;
+        // the block's first point by binary search, then forward to the
+        // thread's own grid (a block spans few grid boundaries)
+        int gi = bfind_point(a.g, a.G, b0);
+        while (gi + 1 < a.G && a.g[gi + 1].p0 <= gid) gi++;
+        const BXf d = a.g[gi];
         long r = gid - d.p0;
         const int o2 = (int)(r % d.no[2]);
         r /= d.no[2];
