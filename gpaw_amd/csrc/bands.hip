@@ -9,7 +9,7 @@
 //    are reduced in a fixed order (bit-reproducible).
 //  * the Cholesky factor and its inverse come from rocSOLVER (zpotrf,
 //    ztrtri) on the column-major view of the row-major matrix, and the
-//    rotation Psi <- C Psi is one rocBLAS ztrmm.
+//    rotation Psi <- C Psi is k_rot (FP64 MFMA, the lower triangle only).
 #include "bands.h"
 
 #include <rocblas/rocblas.h>
@@ -23,6 +23,15 @@ constexpr int GT = 128;  // output tile
 constexpr int GK = 16;   // inner points per LDS stage
 
 typedef double __attribute__((ext_vector_type(4))) d4;
+
+// consecutive logical blocks (the band tiles of one point tile share its
+// psi columns) on one XCD; hardware deals blocks to the 8 XCDs in turn
+__device__ inline int xcd_remap_rot(int h, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int x = h & 7, k = h >> 3;
+    return x * q + min(x, r) + k;
+}
 
 struct GramArgs {
     const double2* A;
@@ -145,6 +154,119 @@ __global__ void __launch_bounds__(512) k_gram(GramArgs a)
                 const int col = wc * 64 + c * 16 + li;
                 out[row * GT + col] = make_double2(accr[r][c][q],
                                                    acci[r][c][q]);
+            }
+}
+
+// Rotation out[i][x] = sum_{j <= i} C[i][j] psi[j][x] with C lower
+// triangular (row-major n x n; entries above the diagonal are ignored) and
+// psi n rows of X points: 128 bands x 128 points per block, K = bands up to
+// the tile's last row, the same MFMA tiling as k_gram (A = C, B = psi
+// without conjugation).
+struct RotArgs {
+    const double2* C;
+    int n;
+    const double2* psi;
+    long ldpsi;
+    long X;
+    double2* out;
+    long ldout;
+    int nti;   // band tiles
+};
+
+__global__ void __launch_bounds__(512) k_rot(RotArgs a)
+{
+    __shared__ double sCr[GK][GT], sCi[GK][GT], sPr[GK][GT], sPi[GK][GT];
+    const int L = xcd_remap_rot(blockIdx.x, gridDim.x);
+    const int bt = L % a.nti;
+    const long xt = L / a.nti;
+    const int i0 = bt * GT;
+    const long x0 = xt * GT;
+    const int kend = min(a.n, i0 + GT);
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    d4 accr[2][4], acci[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            accr[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    // C tile: 128 rows x 16 columns, 4 per thread; psi tile: 16 rows x 128
+    // points, 4 consecutive points per thread
+    const int crow = t >> 2, cq = (t & 3) * 4;
+    const int prow = t >> 5, pq = (t & 31) * 4;
+    const int gi = i0 + crow;
+    double2 vc[4], vp[4];
+    auto fetch = [&](int ks) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = ks + cq + q;
+            vc[q] = (gi < a.n && j < kend && j <= gi)
+                        ? a.C[(long)gi * a.n + j]
+                        : make_double2(0.0, 0.0);
+            const int jp = ks + prow;
+            const long x = x0 + pq + q;
+            vp[q] = (jp < kend && x < a.X) ? a.psi[(long)jp * a.ldpsi + x]
+                                           : make_double2(0.0, 0.0);
+        }
+    };
+    fetch(0);
+    for (int ks = 0; ks < kend; ks += GK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sCr[cq + q][crow] = vc[q].x;
+            sCi[cq + q][crow] = vc[q].y;
+            sPr[prow][pq + q] = vp[q].x;
+            sPi[prow][pq + q] = vp[q].y;
+        }
+        __syncthreads();
+        if (ks + GK < kend) fetch(ks + GK);
+#pragma unroll
+        for (int kk = 0; kk < GK / 4; kk++) {
+            const int kl = kk * 4 + lk;
+            double ar[2], ai[2], nai[2], br[4], bi[4];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                ar[r] = sCr[kl][wr * 32 + r * 16 + li];
+                ai[r] = sCi[kl][wr * 32 + r * 16 + li];
+                nai[r] = -ai[r];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                br[c] = sPr[kl][wc * 64 + c * 16 + li];
+                bi[c] = sPi[kl][wc * 64 + c * 16 + li];
+            }
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ar[r], br[c], accr[r][c], 0, 0, 0);
+                    accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        nai[r], bi[c], accr[r][c], 0, 0, 0);
+                    acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ar[r], bi[c], acci[r][c], 0, 0, 0);
+                    acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        ai[r], br[c], acci[r][c], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = i0 + wr * 32 + r * 16 + lk + 4 * q;
+                const long col = x0 + wc * 64 + c * 16 + li;
+                if (row < a.n && col < a.X)
+                    a.out[(long)row * a.ldout + col] =
+                        make_double2(accr[r][c][q], acci[r][c][q]);
             }
 }
 
@@ -280,16 +402,24 @@ int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
                                      (rocblas_double_complex*)S, n,
                                      (rocblas_int*)w.info),
                     "rocsolver_ztrtri"));
-    // Psi^T (X x n, column-major) times the upper factor: psi_i <- C_ij psi_j
-    const rocblas_double_complex one = {1.0, 0.0};
-    GFD_TRY(blas_ok(rocblas_ztrmm_64(h, rocblas_side_right, rocblas_fill_upper,
-                                     rocblas_operation_none,
-                                     rocblas_diagonal_non_unit, X, n, &one,
-                                     (const rocblas_double_complex*)S, n,
-                                     (const rocblas_double_complex*)psi,
-                                     ldpsi, (rocblas_double_complex*)out,
-                                     ldout),
-                    "rocblas_ztrmm"));
+    // psi_i <- sum_{j <= i} C_ij psi_j (C = L^-1 in the row-major lower
+    // triangle of S)
+    RotArgs r;
+    r.C = S;
+    r.n = n;
+    r.psi = psi;
+    r.ldpsi = ldpsi;
+    r.X = X;
+    r.out = out;
+    r.ldout = ldout;
+    r.nti = (n + GT - 1) / GT;
+    const long nb = (long)r.nti * ((X + GT - 1) / GT);
+    if (nb > 0x7fffffffL) {
+        set_error("cholesky_orth: too many blocks");
+        return ENOTSUP_;
+    }
+    hipLaunchKernelGGL(k_rot, dim3((unsigned)nb), dim3(512), 0, s, r);
+    GFD_LAUNCH_CHECK();
     return OK;
 }
 
