@@ -133,11 +133,30 @@ def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
     a = phi - phi.mean()
     b = phi64 - phi64.mean()
     err = float((a - b).abs().max() / b.abs().max())
+    # the FFT solver on the same problem (exact for the FD operator)
+    from math import pi
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.fft_poisson import FastPoissonSolver
+    fs = FastPoissonSolver(nn=3)
+    fs.set_grid_descriptor(gd)
+    pf = torch.zeros_like(drho)
+    fs.solve_neutral(pf, drho)
+    tf = []
+    for _ in range(5):
+        ms, _ = wall(lambda: fs.solve_neutral(pf, drho))
+        tf.append(ms)
+    r = torch.empty_like(drho)
+    relf = (Laplace(gd, -0.25 / pi, 3).residual(pf, drho, r, norm=True) /
+            rn2)**0.5
+    c = pf - pf.mean()
+    errf = float((c - b).abs().max() / b.abs().max())
     return {'fp64_ms': round(min(t64), 3), 'fp64_vcycles': n64,
             'fp64_relres': rel64, 'mixed_ms': round(min(tmx), 3),
             'mixed_refinements': nmx, 'mixed_relres': relmx,
             'speedup_mixed_vs_fp64': round(min(t64) / min(tmx), 3),
             'mixed_err_vs_fp64': err,
+            'fft_ms': round(min(tf), 3), 'fft_relres': relf,
+            'fft_err_vs_fp64': errf,
             'levels': s.levels + 1,
             'mixed_cycle': 'V(2,2) FP32, w0=%g, %d per step, 4 grid-wide '
                            'levels + LDS coarse solve' % (w0, inner)}

@@ -19,7 +19,8 @@ ARCH = 'gfx950'
 # longest compiles first (the pool runs them in this order)
 SOURCES = ['rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
            'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
-           'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip']
+           'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip',
+           'fftpoisson.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',
@@ -49,7 +50,7 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', f'--offload-arch={ARCH}', '-o', tmp]
     cmd += objs
-    cmd += ['-L/opt/rocm/lib', '-lrocsolver', '-lrocblas',
+    cmd += ['-L/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-lhipfft',
             '-Wl,-rpath,/opt/rocm/lib']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

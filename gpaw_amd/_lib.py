@@ -70,6 +70,12 @@ SIGNATURES = {
     'gfd_batch_poisson_solve_mixed': (_i, [_vp, ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), _d, _i, _i,
                                            _i, _i, _i, _ip, _dp, _vp]),
+    'gfd_fft_poisson_spectrum_shape': (_l, [ctypes.POINTER(_l), _ip,
+                                            ctypes.POINTER(_l)]),
+    'gfd_fft_poisson_new': (_i, [ctypes.POINTER(_l), _ip, _dp,
+                                 ctypes.POINTER(_vp)]),
+    'gfd_fft_poisson_free': (None, [_vp]),
+    'gfd_fft_poisson_solve': (_i, [_vp, _vp, _vp, _vp]),
     'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
     'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),

@@ -341,7 +341,11 @@ class FDPoissonSolver(BasePoissonSolver):
 
 
 def PoissonSolver(name='fd', **kwargs):
+    if name == 'fast':
+        from .fft_poisson import FastPoissonSolver
+        return FastPoissonSolver(**kwargs)
     if name != 'fd':
         raise NotImplementedError(f'Poisson solver {name!r}: only the FD '
-                                  'multigrid solver is part of this library')
+                                  'multigrid and FFT solvers are part of '
+                                  'this library')
     return FDPoissonSolver(**kwargs)

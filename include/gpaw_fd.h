@@ -227,6 +227,27 @@ int gfd_batch_poisson_solve_mixed(gfd_batch_poisson* b, double* const* phis,
                                   int max_outer, int use_graph, int* n_outer,
                                   double* relres, void* stream);
 
+/* ------------------------------------------------------- FFT Poisson
+ * FastPoissonSolver (gpaw/poisson.py:846-1028): the FD operator is
+ * diagonal in the Fourier basis of periodic axes and the DST-I basis of
+ * zero-boundary axes.  n: grid points (interior points on zero axes);
+ * zero_bc[c]: axis c has zero boundaries (odd extension to 2 (n + 1)
+ * points); inv_lambda: host array of the inverse eigenvalues on the
+ * extended grid's real-to-complex half spectrum, C order, shape
+ * (m0, m1, m2/2 + 1) as gfd_fft_poisson_spectrum_shape returns (0 where
+ * the eigenvalue vanishes).  hipFFT plans and work arrays are made here. */
+typedef struct gfd_fft_poisson gfd_fft_poisson;
+
+long gfd_fft_poisson_spectrum_shape(const long n[3], const int zero_bc[3],
+                                    long m_out[3]);
+int gfd_fft_poisson_new(const long n[3], const int zero_bc[3],
+                        const double* inv_lambda, gfd_fft_poisson** out);
+void gfd_fft_poisson_free(gfd_fft_poisson* f);
+
+/* phi = L^-1 rho on the grid (float64 device arrays of shape n). */
+int gfd_fft_poisson_solve(gfd_fft_poisson* f, const double* rho,
+                          double* phi, void* stream);
+
 /* ---------------------------------------------------------------- bands
  * FD eigensolver step (complex128 bands, row-major nbands x npts with row
  * stride ld points). */
