@@ -76,6 +76,8 @@ SIGNATURES = {
                                  ctypes.POINTER(_vp)]),
     'gfd_fft_poisson_free': (None, [_vp]),
     'gfd_fft_poisson_solve': (_i, [_vp, _vp, _vp, _vp]),
+    'gfd_h_residual': (_i, [_vp, _vp, _vp, _vp, _vp, _l, _dp, _vp]),
+    'gfd_band_dots': (_i, [_i, _vp, _vp, _i, _l, _l, _d, _vp, _vp]),
     'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
     'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),

@@ -7,6 +7,9 @@ matrices (FP64 MFMA) and Cholesky orthonormalisation.
   (gpaw/core/arrays.py:183-262): S[i, j] = dv sum psi_i conj(psi_j)
 - ``subspace_matrix``: matrix_elements(psit, Htpsit):
   H[i, j] = dv sum conj(psi_i) (H psi)_j
+- ``apply_hamiltonian(..., eps=)``: the residuals (H - eps_n) psi_n of
+  calculate_residuals (gpaw/new/pwfd/eigensolver.py:197-227), grid part
+- ``band_dots``: dv Re <a_n|b_n> per band (band energies)
 - ``orthonormalize``: gpaw/new/pwfd/wave_functions.py:191-246 for the grid
   part (no PAW projections): S -> L = chol(S), C = L^-1, psi <- C psi
   (Matrix.invcholesky, gpaw/core/matrix.py:382-417)
@@ -26,8 +29,11 @@ def _npts(psi):
     return int(np.prod(psi.shape[1:]))
 
 
-def apply_hamiltonian(kin, vt, psi, out, phase_cd=None):
-    """out = kin(psi) + vt psi (vt: real grid, float64 device tensor)."""
+def apply_hamiltonian(kin, vt, psi, out, phase_cd=None, eps=None):
+    """out = kin(psi) + vt psi (vt: real grid, float64 device tensor); with
+    eps (float64 device tensor, one value per band) the residuals
+    out_n = (kin + vt - eps_n) psi_n in the same pass
+    (gpaw/new/pwfd/eigensolver.py:197-227)."""
     if tuple(psi.shape[-3:]) != kin.shape or tuple(vt.shape) != kin.shape:
         raise ValueError('grid shape mismatch')
     if psi.shape != out.shape or not psi.is_contiguous() or \
@@ -40,9 +46,32 @@ def apply_hamiltonian(kin, vt, psi, out, phase_cd=None):
         keep, ph = _lib.phases_arg(np.ones((3, 2), complex)
                                    if phase_cd is None else phase_cd)
     nb = int(np.prod(psi.shape[:-3]))
+    if eps is not None:
+        if eps.dtype != torch.float64 or eps.numel() != nb or \
+                not eps.is_contiguous():
+            raise ValueError('eps: contiguous float64, one per band')
+        _lib.check(_lib.load().gfd_h_residual(
+            kin._handle(), ptr(vt), ptr(eps), ptr(psi), ptr(out), nb, ph,
+            stream_handle()), 'apply_hamiltonian(eps)')
+        return out
     _lib.check(_lib.load().gfd_h_apply(kin._handle(), ptr(vt), ptr(psi),
                                        ptr(out), nb, ph, stream_handle()),
                'apply_hamiltonian')
+    return out
+
+
+def band_dots(a, b, dv, out=None):
+    """out[n] = dv Re <a_n|b_n> for (nbands, ...) float64 / complex128
+    device tensors."""
+    nb = a.shape[0]
+    npts = _npts(a)
+    if b.shape != a.shape or b.dtype != a.dtype:
+        raise ValueError('a and b: same shape and dtype')
+    if out is None:
+        out = torch.empty(nb, dtype=torch.float64, device=a.device)
+    _lib.check(_lib.load().gfd_band_dots(
+        int(a.dtype == torch.complex128), ptr(a), ptr(b), nb, npts, npts,
+        float(dv), ptr(out), stream_handle()), 'band_dots')
     return out
 
 

@@ -1352,6 +1352,38 @@ extern "C" int gfd_h_apply(gfd_operator* kin, const double* vpot,
                    (hipStream_t)stream, nullptr);
 }
 
+extern "C" int gfd_h_residual(gfd_operator* kin, const double* vpot,
+                              const double* eps, const void* psi, void* r,
+                              long nbands, const double* phases,
+                              void* stream)
+{
+    if (!kin || !vpot || !eps || nbands < 0 ||
+        (nbands > 0 && (!psi || !r)) || psi == r) {
+        set_error("gfd_h_residual: bad arguments");
+        return EINVAL_;
+    }
+    if (nbands == 0) return OK;
+    EpiArgs ea{};
+    ea.c0 = centre_coef(kin);
+    ea.pot = vpot;
+    ea.shift = eps;
+    return op_pass(kin, kin->dtype(), psi, r, nbands,
+                   kin->real ? nullptr : phases, EPI_POT, ea,
+                   (hipStream_t)stream, nullptr);
+}
+
+extern "C" int gfd_band_dots(int is_complex, const void* a, const void* b,
+                             int nbands, long npts, long ld, double dv,
+                             double* out, void* stream)
+{
+    if (!a || !b || !out || nbands < 0 || npts < 0 || ld < npts) {
+        set_error("gfd_band_dots: bad arguments");
+        return EINVAL_;
+    }
+    return band_dots(is_complex ? 1 : 0, a, b, nbands, npts, ld, dv, out,
+                     (hipStream_t)stream);
+}
+
 extern "C" int gfd_band_overlap(const void* psi, int nbands, long npts,
                                 long ld, double dv, void* S, int lower_only,
                                 void* stream)

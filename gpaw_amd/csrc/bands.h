@@ -22,6 +22,11 @@ int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
          int M, int N, long K, double alpha, int conj, int lower_only,
          double2* out, long ldo, hipStream_t s);
 
+// out[n] = dv Re sum_x conj(A_n(x)) B_n(x), dtype 0 f64 / 1 c128, rows of
+// npts points at stride ld; out a device array of nbands doubles.
+int band_dots(int dtype, const void* A, const void* B, int nbands, long npts,
+              long ld, double dv, double* out, hipStream_t s);
+
 // Lower Cholesky L of S (row-major, lower part used), C = L^-1 in S, and
 // out_i = sum_j C_ij psi_j for the n rows of psi (X points each).
 int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,

@@ -295,6 +295,52 @@ k_gram_reduce(const double2* __restrict__ partial, int nchunks, int ntiles,
         make_double2(alpha * sr, conj ? -alpha * si : alpha * si);
 }
 
+// out[n] = dv Re sum_x conj(a_n(x)) b_n(x): one block per band, a fixed
+// order of partial sums (bit-reproducible)
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_band_dot(const T* __restrict__ A, const T* __restrict__ B, long npts,
+           long ld, double dv, double* __restrict__ out)
+{
+    const T* a = A + (long)blockIdx.x * ld;
+    const T* b = B + (long)blockIdx.x * ld;
+    double s = 0.0;
+    for (long x = threadIdx.x; x < npts; x += 1024) {
+        if constexpr (sizeof(T) == 16) {
+            const double2 u = *reinterpret_cast<const double2*>(a + x);
+            const double2 v = *reinterpret_cast<const double2*>(b + x);
+            s = fma(u.x, v.x, s);
+            s = fma(u.y, v.y, s);
+        } else {
+            s = fma((double)a[x], (double)b[x], s);
+        }
+    }
+    __shared__ double red[1024];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = dv * red[0];
+}
+
+int band_dots(int dtype, const void* A, const void* B, int nbands, long npts,
+              long ld, double dv, double* out, hipStream_t s)
+{
+    if (nbands <= 0) return OK;
+    if (dtype == 1)
+        hipLaunchKernelGGL(k_band_dot<double2>, dim3(nbands), dim3(1024), 0,
+                           s, (const double2*)A, (const double2*)B, npts, ld,
+                           dv, out);
+    else
+        hipLaunchKernelGGL(k_band_dot<double>, dim3(nbands), dim3(1024), 0,
+                           s, (const double*)A, (const double*)B, npts, ld,
+                           dv, out);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
 int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
          int M, int N, long K, double alpha, int conj, int lower_only,
          double2* out, long ldo, hipStream_t s)

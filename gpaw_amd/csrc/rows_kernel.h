@@ -50,6 +50,7 @@ struct XrArgs {
     BC bc;
     double* partial;
     const double* pot;
+    const double* shift;
     int nt;              // non-temporal hints (xr_nt_flags)
 };
 
@@ -363,6 +364,13 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                         for (int p = 0; p < PZ; p++)
                             r.v[p] = fmac(fmac(acc[p], (S)a.c0, q[u][j].v[p]),
                                           (S)pv[p], q[u][j].v[p]);
+                        if (a.shift) {
+                            // residual: - eps_b psi
+                            const S e = -(S)a.shift[b];
+#pragma unroll
+                            for (int p = 0; p < PZ; p++)
+                                r.v[p] = fmac(r.v[p], e, q[u][j].v[p]);
+                        }
                     } else {
 #pragma unroll
                         for (int p = 0; p < PZ; p++)
@@ -425,6 +433,7 @@ static int launch_xr_t(const AxisStencil& st, const XmTile& tl,
     a.bc = bc;
     a.partial = ea.partial;
     a.pot = ea.pot;
+    a.shift = ea.shift;
     a.nt = xr_nt_flags((long)n[0] * n[1] * n[2] * nin * (long)sizeof(T));
     const int TZT = n[2] / PZ;
     const size_t shm =

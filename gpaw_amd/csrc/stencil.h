@@ -16,7 +16,8 @@ enum Epi : int {
     EPI_RELAX = 1,   // out = (1 - w) v + w (aux - acc) / c0        (Jacobi)
     EPI_RESID = 2,   // out = (c0 v + acc) - aux          [+ sum out^2]
     EPI_UPDATE = 3,  // out = +-(v - s ((c0 v + acc) - aux))
-    EPI_POT = 4,     // out = (c0 v + acc) + pot v, pot real per grid point
+    EPI_POT = 4,     // out = (c0 v + acc) + pot v [- shift[b] v], pot real
+                     // per grid point, shift one real per grid of the batch
     EPI_UPDW = 5,    // EPI_UPDATE on FP32 data with aux and out in FP64
     EPI_RESN = 6,    // EPI_RESID on FP64 data, out narrowed to FP32
                      // (shared by the grids of a batch): H psi
@@ -30,6 +31,7 @@ struct EpiArgs {
     int negate;        // UPDATE: store -(...)
     double* partial;   // RESID: per-block sum of |out|^2, or 0
     const double* pot; // POT: real potential of one grid
+    const double* shift = nullptr;  // POT: per-grid eigenvalue (residual)
 };
 
 // Axis-only (central-difference) stencil in per-axis form.

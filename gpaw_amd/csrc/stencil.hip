@@ -300,6 +300,7 @@ struct GenArgs {
     BC bc;
     double* partial;
     const double* pot;
+    const double* shift;
 };
 
 constexpr int GEN_BLOCK = 256;
@@ -339,8 +340,9 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
         const T v = g[((long)i0 * a.n1 + i1) * a.n2 + i2];
         if constexpr (EPI == EPI_POT) {
             typedef typename ScalarOf<T>::type S;
-            a.out[gid] = fmac(fmac(acc, (S)a.c0, v), (S)a.pot[gid - b * a.ng],
-                              v);
+            T r = fmac(fmac(acc, (S)a.c0, v), (S)a.pot[gid - b * a.ng], v);
+            if (a.shift) r = fmac(r, -(S)a.shift[b], v);
+            a.out[gid] = r;
         } else if constexpr (EPI == EPI_RESN) {
             ((float*)(void*)a.out)[gid] = (float)epilogue<T, EPI_RESID>(
                 acc, v, a.aux, gid, a.c0, a.w, a.s, a.negate, nrm);
@@ -624,6 +626,7 @@ static int launch_gen_t(const GeneralStencil& st, const int n[3],
     a.bc = bc;
     a.partial = ea.partial;
     a.pot = ea.pot;
+    a.shift = ea.shift;
     const long nb = (a.total + GEN_BLOCK - 1) / GEN_BLOCK;
     if (nb <= 0) return OK;
     if (nb > 0x7fffffffL) {

@@ -258,6 +258,21 @@ int gfd_fft_poisson_solve(gfd_fft_poisson* f, const double* rho,
 int gfd_h_apply(gfd_operator* kin, const double* vpot, const void* psi,
                 void* hpsi, long nbands, const double* phases, void* stream);
 
+/* Residuals r_n = (kin + vpot - eps[n]) psi_n of the grid part of
+ * calculate_residuals (gpaw/new/pwfd/eigensolver.py:197-227, the
+ * `axpy(-e, p, r)` after apply_local_potential) in one stencil pass; eps is
+ * a device array of nbands doubles. */
+int gfd_h_residual(gfd_operator* kin, const double* vpot, const double* eps,
+                   const void* psi, void* r, long nbands,
+                   const double* phases, void* stream);
+
+/* out[n] = dv Re sum_x conj(a_n(x)) b_n(x) for nbands rows of npts points
+ * (stride ld), float64 (is_complex 0) or complex128; out is a device array
+ * of nbands doubles.  The band energies <psi_n|H|psi_n> of the eigensolver
+ * step (gpaw/new/pwfd/eigensolver.py). */
+int gfd_band_dots(int is_complex, const void* a, const void* b, int nbands,
+                  long npts, long ld, double dv, double* out, void* stream);
+
 /* S[i][j] = dv sum_x psi_i(x) conj(psi_j(x)), row-major nbands x nbands:
  * the matrix_elements(psit, psit, cc=True) of gpaw/core/arrays.py:183-262
  * used by orthonormalize (gpaw/new/pwfd/wave_functions.py:223-226).  With

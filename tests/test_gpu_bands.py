@@ -9,8 +9,8 @@ import pytest
 import scipy.linalg as sla
 
 import oracle
-from gpaw_amd.bands import (apply_hamiltonian, orthonormalize, overlap,
-                            subspace_matrix)
+from gpaw_amd.bands import (apply_hamiltonian, band_dots, orthonormalize,
+                            overlap, subspace_matrix)
 from gpaw_amd.fd_operators import Laplace
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpu_util import dev, host, say
@@ -80,3 +80,42 @@ def test_h_apply():
         say('h apply', dtype.__name__)
         apply_hamiltonian(kin, dev(vt), d, out, ph)
         assert relerr(host(out), ref) < 1e-13
+
+
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_h_residual_and_band_dots(dtype):
+    """The fused residual (H - eps_n) psi_n against the oracle stencil plus
+    numpy, and the per-band energies against numpy's vdot."""
+    import torch
+    rng = np.random.default_rng(5)
+    gd = GridDescriptor((12, 16, 10), (2.0, 3.0, 2.5),
+                        pbc_c=(True, False, True))
+    nb = 5
+    kin = Laplace(gd, -0.5, 3, dtype)
+    psi = rng.standard_normal((nb,) + tuple(gd.n_c))
+    if dtype == complex:
+        psi = psi + 1j * rng.standard_normal(psi.shape)
+    vt = rng.standard_normal(tuple(gd.n_c))
+    eps = rng.standard_normal(nb)
+    ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+    hpsi = oracle.fd_apply(kin.coef_p, kin.offset_pc, gd.n_c,
+                           gd.neighbor_cd, psi, ph) + psi * vt
+    ref = hpsi - eps[:, None, None, None] * psi
+    d = dev(psi)
+    out = d.new_empty(d.shape)
+    say('h residual', dtype.__name__)
+    apply_hamiltonian(kin, dev(vt), d, out, ph,
+                      eps=torch.tensor(eps, dtype=torch.float64,
+                                       device=d.device))
+    assert relerr(host(out), ref) < 1e-13
+    # band energies: dv Re <psi_n|H psi_n>
+    dv = gd.dv
+    e_ref = np.array([dv * np.vdot(psi[n], hpsi[n]).real for n in range(nb)])
+    e = host(band_dots(d, dev(hpsi), dv))
+    assert np.abs(e - e_ref).max() < 1e-12 * np.abs(e_ref).max()
+    # residual orthogonal to psi_n when eps_n is its Rayleigh quotient
+    norms = np.array([dv * np.vdot(psi[n], psi[n]).real for n in range(nb)])
+    rq = torch.tensor(e_ref / norms, dtype=torch.float64, device=d.device)
+    apply_hamiltonian(kin, dev(vt), d, out, ph, eps=rq)
+    ov = host(band_dots(d, out, dv))
+    assert np.abs(ov).max() < 1e-11 * np.abs(e_ref).max()
