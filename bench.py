@@ -11,8 +11,9 @@ Poisson solve of the 48^3 four-Gaussian problem (eps 2e-10), the kinetic
 preconditioner on 32 complex bands, the 192^3 Poisson time to a 1e-10
 relative residual (all-FP64 V-cycles against FP32 V-cycles inside FP64
 refinement), the 48 uneven Poisson problems solved as one batch, and the FD
-eigensolver step on 1024 complex bands of 128^3 at 4 k-points (H apply,
-preconditioner, FP64 MFMA overlap, Cholesky orthonormalisation).
+eigensolver step on 1024 complex bands of 128^3 at 4 k-points (fused
+residuals, the RMM-DIIS block step with the FP32 preconditioner, FP64 MFMA
+overlap, Cholesky orthonormalisation).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--no-stages] [--no-eigen]
@@ -230,16 +231,18 @@ def stage_uneven_48(dev, tol=1e-10):
 
 
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
-    """One FD eigensolver step per k-point (4 Bloch k-points) on nbands
+    """One RMM-DIIS iteration per k-point (4 Bloch k-points) on nbands
     complex bands of 128^3: the residuals R = (-1/2 lap(nn=3) + V - eps_n)
-    psi_n in one fused stencil pass (eps_n the band energies), the
-    multigrid kinetic preconditioner on R, S = dv psi psi^H (FP64 MFMA,
-    lower tiles) and psi <- chol(S)^-1 psi."""
+    psi_n in one fused stencil pass (eps_n the band energies), the block
+    step (two FP32 multigrid preconditioner applications, one more fused
+    residual, the step lengths and the updates), S = dv psi psi^H (FP64
+    MFMA, lower tiles) and psi <- chol(S)^-1 psi."""
     import torch
     from gpaw_amd import workloads
     from gpaw_amd.bands import apply_hamiltonian, band_dots, orthonormalize
     from gpaw_amd.fd_operators import Laplace
     from gpaw_amd.preconditioner import Preconditioner
+    from gpaw_amd.rmmdiis import BlockStep
     gd = workloads.eigen_128_gd()
     phases = workloads.eigen_128_kpoints()
     vt = torch.from_numpy(workloads.eigen_128_potential(gd)).to(dev)
@@ -251,12 +254,12 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     psi = torch.randn(shape, dtype=torch.complex128, device=dev,
                       generator=g)
     hpsi = torch.empty_like(psi)
-    pre = torch.empty_like(psi)
     S = torch.zeros((nbands, nbands), dtype=torch.complex128, device=dev)
     # band energies eps_n = <psi_n|H|psi_n> / <psi_n|psi_n> (first k-point)
     apply_hamiltonian(kin, vt, psi, hpsi, phases[0])
     eps = band_dots(psi, hpsi, gd.dv) / band_dots(psi, psi, gd.dv)
     ms_dots, _ = timed(lambda: band_dots(psi, hpsi, gd.dv), 3, 1)
+    rmm = BlockStep(kin, vt, P)
 
     class Kpt:
         phase_cd = None
@@ -264,17 +267,21 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
     acc = np.zeros(4)
 
+    bufs = [psi, hpsi]
+
     def step(record):
         for k in range(len(phases)):
+            psi, R = bufs
             Kpt.phase_cd = phases[k]
             ev[0].record()
-            apply_hamiltonian(kin, vt, psi, hpsi, phases[k], eps=eps)
+            apply_hamiltonian(kin, vt, psi, R, phases[k], eps=eps)
             ev[1].record()
-            P(hpsi, Kpt, out=pre)
+            rmm(psi, R, eps, phases[k])
             ev[2].record()
-            # orthonormalised bands land in hpsi (free after the
-            # preconditioner); S is computed inside orthonormalize
-            orthonormalize(psi, gd.dv, out=hpsi, S=S)
+            # orthonormalised bands land in R (free after the block step);
+            # S is computed inside orthonormalize
+            orthonormalize(psi, gd.dv, out=R, S=S)
+            bufs.reverse()
             ev[3].record()
             torch.cuda.synchronize()
             if record:
@@ -290,16 +297,25 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
         step(True)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    # the overlap alone, for its MFMA rate
+    psi, R = bufs
+    # the overlap and the preconditioner alone
     from gpaw_amd.bands import overlap
     ms_s, _ = timed(lambda: overlap(psi, gd.dv, True, S), 3, 1)
+    Kpt.phase_cd = phases[-1]
+    ms_pc, _ = timed(lambda: P(R, Kpt, out=rmm._PR), 3, 1)
     # checks at full size: the rotated bands are orthonormal (their own
-    # overlap is the identity); the FP32 preconditioner against the FP64
-    # one on 8 bands
-    overlap(hpsi, gd.dv, False, S)
+    # overlap is the identity); one block step lowers the residual norm
+    # (R + lambda dR is the residual of psi + lambda PR); the FP32
+    # preconditioner against the FP64 one on 8 bands
+    overlap(psi, gd.dv, False, S)
     orth_err = float((S - torch.eye(nbands, dtype=S.dtype,
                                     device=S.device)).abs().max())
-    Kpt.phase_cd = phases[-1]
+    apply_hamiltonian(kin, vt, psi, R, phases[-1])
+    eps = band_dots(psi, R, gd.dv)
+    apply_hamiltonian(kin, vt, psi, R, phases[-1], eps=eps)
+    r0 = float(band_dots(R, R, gd.dv).sum())
+    rmm(psi, R, eps, phases[-1])
+    r1 = float(band_dots(rmm._dR, rmm._dR, gd.dv).sum())
     P64 = Preconditioner(gd, kin, complex, block=8)
     ref = P64(psi[:8], Kpt)
     got = P(psi[:8], Kpt)
@@ -310,6 +326,7 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     herk_flop = nbands * (nbands + 1) / 2 * X * 8
     h_bytes = nbands * X * 32
     per = acc / (steps * nk)
+    upd_bytes = nbands * X * 16 * (6 + 3)
     return {'ms_per_step_4k': round(wall, 2),
             'h_residual_ms_per_k': round(per[0], 3),
             'h_residual_hbm_frac': round(h_bytes / per[0] / 1e6 /
@@ -317,7 +334,12 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
             'band_dots_ms': round(ms_dots, 3),
             'band_dots_hbm_frac': round(h_bytes / ms_dots / 1e6 /
                                         HBM_PEAK_GBS, 4),
-            'precond_ms_per_k': round(per[1], 3),
+            'rmm_block_step_ms_per_k': round(per[1], 3),
+            'precond_ms': round(ms_pc, 3),
+            'rmm_step_other_ms': round(per[1] - 2 * ms_pc - per[0] -
+                                       ms_dots, 3),
+            'rmm_update_bytes_gb': round(upd_bytes / 1e9, 1),
+            'rmm_residual_ratio': r1 / r0,
             'overlap_orth_ms_per_k': round(per[2], 3),
             'overlap_ms': round(ms_s, 3),
             'overlap_mfma_tflops': round(herk_flop / ms_s / 1e9, 2),
@@ -325,8 +347,9 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
             'orthonormality_err': orth_err,
             'precond_fp32_relerr_vs_fp64': pre_err,
             'bands': nbands, 'kpoints': nk, 'grid': [128, 128, 128],
-            'note': 'FP32 preconditioner applied to the residuals; '
-                    'residual, overlap and Cholesky in FP64'}
+            'note': 'RMM-DIIS (diis_steps=1) with the FP32 preconditioner; '
+                    'residuals, step lengths, overlap and Cholesky in '
+                    'FP64'}
 
 
 def cpu_baseline(gd, op_coefs, op_offsets, nbands_sample):

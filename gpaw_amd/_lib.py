@@ -78,6 +78,9 @@ SIGNATURES = {
     'gfd_fft_poisson_solve': (_i, [_vp, _vp, _vp, _vp]),
     'gfd_h_residual': (_i, [_vp, _vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_dots': (_i, [_i, _vp, _vp, _i, _l, _l, _d, _vp, _vp]),
+    'gfd_rmm_lambda': (_i, [_i, _vp, _vp, _i, _l, _l, _vp, _vp]),
+    'gfd_band_update': (_i, [_i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i, _l, _l,
+                             _vp]),
     'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
     'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),

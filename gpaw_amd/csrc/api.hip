@@ -1384,6 +1384,33 @@ extern "C" int gfd_band_dots(int is_complex, const void* a, const void* b,
                      (hipStream_t)stream);
 }
 
+extern "C" int gfd_rmm_lambda(int is_complex, const void* dR, const void* R,
+                              int nbands, long npts, long ld, double* lam,
+                              void* stream)
+{
+    if (!dR || !R || !lam || nbands < 0 || npts < 0 || ld < npts) {
+        set_error("gfd_rmm_lambda: bad arguments");
+        return EINVAL_;
+    }
+    return rmm_lambda(is_complex ? 1 : 0, dR, R, nbands, npts, ld, lam,
+                      (hipStream_t)stream);
+}
+
+extern "C" int gfd_band_update(int is_complex, int mode, const double* alpha,
+                               double scale, const void* x, void* y,
+                               const void* v, void* w, int nbands, long npts,
+                               long ld, void* stream)
+{
+    if (!alpha || !x || !y || (mode != 0 && mode != 1) ||
+        (mode == 1 && (!v || !w || w == y)) || nbands < 0 || npts < 0 ||
+        ld < npts) {
+        set_error("gfd_band_update: bad arguments");
+        return EINVAL_;
+    }
+    return band_update(is_complex ? 1 : 0, mode, alpha, scale, x, y, v, w,
+                       nbands, npts, ld, (hipStream_t)stream);
+}
+
 extern "C" int gfd_band_overlap(const void* psi, int nbands, long npts,
                                 long ld, double dv, void* S, int lower_only,
                                 void* stream)

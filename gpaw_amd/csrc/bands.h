@@ -27,6 +27,16 @@ int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
 int band_dots(int dtype, const void* A, const void* B, int nbands, long npts,
               long ld, double dv, double* out, hipStream_t s);
 
+// lam[n] = -Re<dR_n|R_n> / <dR_n|dR_n> (0 where dR_n = 0)
+int rmm_lambda(int dtype, const void* dR, const void* R, int nbands,
+               long npts, long ld, double* lam, hipStream_t s);
+
+// c_n = scale alpha[n]; mode 0: y_n += c_n x_n; mode 1 also
+// w_n <- v_n + c_n w_n in the same pass.
+int band_update(int dtype, int mode, const double* alpha, double scale,
+                const void* x, void* y, const void* v, void* w, int nbands,
+                long npts, long ld, hipStream_t s);
+
 // Lower Cholesky L of S (row-major, lower part used), C = L^-1 in S, and
 // out_i = sum_j C_ij psi_j for the n rows of psi (X points each).
 int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,

@@ -341,6 +341,132 @@ int band_dots(int dtype, const void* A, const void* B, int nbands, long npts,
     return OK;
 }
 
+// Two real parts of band inner products at once: s0 = Re sum conj(a) b,
+// s1 = Re sum conj(a) a.
+template <typename T>
+__device__ inline void dot2(const T* a, const T* b, long x, double& s0,
+                            double& s1)
+{
+    if constexpr (sizeof(T) == 16) {
+        const double2 u = *reinterpret_cast<const double2*>(a + x);
+        const double2 v = *reinterpret_cast<const double2*>(b + x);
+        s0 = fma(u.x, v.x, fma(u.y, v.y, s0));
+        s1 = fma(u.x, u.x, fma(u.y, u.y, s1));
+    } else {
+        const double u = (double)a[x], v = (double)b[x];
+        s0 = fma(u, v, s0);
+        s1 = fma(u, u, s1);
+    }
+}
+
+// RMM step length per band: lambda_n = -Re<dR_n|R_n> / <dR_n|dR_n>
+// (the dv factors cancel); one block per band, fixed summation order.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_rmm_lambda(const T* __restrict__ dR, const T* __restrict__ R, long npts,
+             long ld, double* __restrict__ lam)
+{
+    const T* a = dR + (long)blockIdx.x * ld;
+    const T* b = R + (long)blockIdx.x * ld;
+    double s0 = 0.0, s1 = 0.0;
+    for (long x = threadIdx.x; x < npts; x += 1024) dot2(a, b, x, s0, s1);
+    __shared__ double r0[1024], r1[1024];
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            r0[threadIdx.x] += r0[threadIdx.x + o];
+            r1[threadIdx.x] += r1[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        lam[blockIdx.x] = r1[0] > 0.0 ? -r0[0] / r1[0] : 0.0;
+}
+
+int rmm_lambda(int dtype, const void* dR, const void* R, int nbands,
+               long npts, long ld, double* lam, hipStream_t s)
+{
+    if (nbands <= 0) return OK;
+    if (dtype == 1)
+        hipLaunchKernelGGL(k_rmm_lambda<double2>, dim3(nbands), dim3(1024),
+                           0, s, (const double2*)dR, (const double2*)R, npts,
+                           ld, lam);
+    else
+        hipLaunchKernelGGL(k_rmm_lambda<double>, dim3(nbands), dim3(1024),
+                           0, s, (const double*)dR, (const double*)R, npts,
+                           ld, lam);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+// Per-band updates with a device coefficient c_n = scale * alpha[n]:
+//   mode 0:  y_n += c_n x_n
+//   mode 1:  y_n += c_n x_n  and  w_n = v_n + c_n w_n   (both RMM updates:
+//            psi += lambda PR, dR <- R + lambda dR, one pass)
+struct BandUpd {
+    const double* alpha;
+    double scale;
+    long npts, ld;
+    int nbands, mode;
+    const void* x;
+    void* y;
+    const void* v;
+    void* w;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_band_update(BandUpd u)
+{
+    const long x = (long)blockIdx.x * 256 + threadIdx.x;
+    if (x >= u.npts) return;
+    for (int n = blockIdx.y; n < u.nbands; n += gridDim.y) {
+        const double c = u.scale * u.alpha[n];
+        const long i = (long)n * u.ld + x;
+        T* y = (T*)u.y;
+        const T* xs = (const T*)u.x;
+        T yv = y[i];
+        const T xv = xs[i];
+        if constexpr (sizeof(T) == 16) {
+            yv.x = fma(c, xv.x, yv.x);
+            yv.y = fma(c, xv.y, yv.y);
+        } else {
+            yv = fma(c, xv, yv);
+        }
+        y[i] = yv;
+        if (u.mode == 1) {
+            T* w = (T*)u.w;
+            const T* vs = (const T*)u.v;
+            T wv = w[i];
+            const T vv = vs[i];
+            if constexpr (sizeof(T) == 16) {
+                wv.x = fma(c, wv.x, vv.x);
+                wv.y = fma(c, wv.y, vv.y);
+            } else {
+                wv = fma(c, wv, vv);
+            }
+            w[i] = wv;
+        }
+    }
+}
+
+int band_update(int dtype, int mode, const double* alpha, double scale,
+                const void* x, void* y, const void* v, void* w, int nbands,
+                long npts, long ld, hipStream_t s)
+{
+    if (nbands <= 0 || npts <= 0) return OK;
+    BandUpd u{alpha, scale, npts, ld, nbands, mode, x, y, v, w};
+    const dim3 grid((unsigned)((npts + 255) / 256),
+                    (unsigned)(nbands < 65535 ? nbands : 65535));
+    if (dtype == 1)
+        hipLaunchKernelGGL(k_band_update<double2>, grid, dim3(256), 0, s, u);
+    else
+        hipLaunchKernelGGL(k_band_update<double>, grid, dim3(256), 0, s, u);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
 int gram(BandWork& w, const double2* A, long lda, const double2* B, long ldb,
          int M, int N, long K, double alpha, int conj, int lower_only,
          double2* out, long ldo, hipStream_t s)

@@ -273,6 +273,21 @@ int gfd_h_residual(gfd_operator* kin, const double* vpot, const double* eps,
 int gfd_band_dots(int is_complex, const void* a, const void* b, int nbands,
                   long npts, long ld, double dv, double* out, void* stream);
 
+/* RMM step length per band, lam[n] = -Re<dR_n|R_n> / <dR_n|dR_n> (0 where
+ * dR_n vanishes): the a_n / b_n of block_step
+ * (gpaw/new/pwfd/rmmdiis.py:150-155) in one pass over dR and R. */
+int gfd_rmm_lambda(int is_complex, const void* dR, const void* R, int nbands,
+                   long npts, long ld, double* lam, void* stream);
+
+/* Per-band updates with c_n = scale * alpha[n] (alpha: device array):
+ * mode 0: y_n += c_n x_n (rmmdiis.py:252-257, the final preconditioned
+ * step); mode 1: y_n += c_n x_n and w_n <- v_n + c_n w_n in one pass
+ * (rmmdiis.py:157-162: psi + lambda PR and R + lambda dR).  w must not
+ * alias y. */
+int gfd_band_update(int is_complex, int mode, const double* alpha,
+                    double scale, const void* x, void* y, const void* v,
+                    void* w, int nbands, long npts, long ld, void* stream);
+
 /* S[i][j] = dv sum_x psi_i(x) conj(psi_j(x)), row-major nbands x nbands:
  * the matrix_elements(psit, psit, cc=True) of gpaw/core/arrays.py:183-262
  * used by orthonormalize (gpaw/new/pwfd/wave_functions.py:223-226).  With

@@ -11,6 +11,8 @@ Python restatements of the reference glue that chains them:
 - ``fd_precond``      gpaw/preconditioner.py:35-93 and
                       c/fd_preconditioner.cpp:221-345
 - ``PoissonVCycle``   gpaw/poisson.py:474-537 (solve_neutral / iterate2)
+- ``rmm_block_step``  gpaw/new/pwfd/rmmdiis.py:123-257 (diis_steps=1, grid
+                      part: no projector terms)
 
 Pinning (what checks each function; see tests/test_oracle_*.py):
 
@@ -23,6 +25,11 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
 - ``PoissonVCycle``: against the same V-cycle driving the reference C
   operators (fixtures: niter and phi), and against the spectral closed form
   of the converged periodic solution.
+- ``rmm_block_step``: by what the step's algebra fixes -- lambda minimises
+  |R + lambda dR| (Re<dR|R + lambda dR> = 0), R + lambda dR is the residual
+  of psi + lambda PR (linearity of H), an exact eigenpair is a fixed point,
+  and on a one-band quadratic model the step equals the closed-form line
+  minimum.
 """
 from __future__ import annotations
 
@@ -318,3 +325,31 @@ class PoissonVCycle:
             phi_ave = float(np.sum(phi.ravel())) / phi.size
             phi -= phi_ave
         return niter
+
+
+def rmm_block_step(kin, vt, precond, psi, R, eig, phases=None,
+                   trial_step=None):
+    """RMM-DIIS block step with diis_steps=1 on the grid part
+    (gpaw/new/pwfd/rmmdiis.py:123-257): precond(x) is the kinetic
+    preconditioner, kin a StencilSpec, vt the local potential, eig the
+    band energies.  Returns (psi_new, intermediates)."""
+    def resid(x):
+        # calculate_residuals without projections
+        # (gpaw/new/pwfd/eigensolver.py:197-227)
+        hx = fd_apply(kin.coef_p, kin.offset_pc, kin.n_c, kin.neighbor_cd,
+                      x, phases) + x * vt
+        return hx - eig.reshape((-1, 1, 1, 1)) * x
+
+    nb = psi.shape[0]
+    PR = precond(R)
+    dR = resid(PR)
+    a = np.array([-np.vdot(dR[n], R[n]).real for n in range(nb)])
+    b = np.array([np.vdot(dR[n], dR[n]).real for n in range(nb)])
+    lam = np.where(b > 0, a / np.where(b > 0, b, 1.0), 0.0)
+    lr = lam.reshape((-1, 1, 1, 1))
+    R1 = R + lr * dR
+    psi1 = psi + lr * PR
+    PR2 = precond(R1)
+    step = lr if trial_step is None else trial_step
+    return psi1 + step * PR2, {'PR': PR, 'dR': dR, 'lam': lam, 'R1': R1,
+                               'psi1': psi1}
