@@ -1510,6 +1510,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         g_force_path = (int)value;
         return OK;
     }
+    if (std::strcmp(key, "xdma") == 0) {
+        xr_set_dma(value != 0);
+        return OK;
+    }
     if (std::strcmp(key, "autotune") == 0) {
         xr_set_autotune(value != 0);
         return OK;
@@ -1540,6 +1544,7 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         t.nseg = (int)((value >> 12) & 0xfff);
         t.py = (int)((value >> 24) & 0xff);
         t.xchunk = (int)((value >> 32) & 0xffff);
+        t.dma = (int)((value >> 48) & 0xf);
         if (t.tz <= 0 || t.nseg <= 0 || t.py <= 0 || t.xchunk <= 0) {
             set_error("gfd_set_tuning: bad xr_tile");
             return EINVAL_;

@@ -30,4 +30,10 @@ int row_launch_c64(const AxisStencil& st, const XmTile& tl, const int n[3],
                    const BC& bc, const void* in, void* out, long nin, int epi,
                    const EpiArgs& ea, hipStream_t stream, int* nblocks_out);
 
+// k_xdma (rows_dma.h): real data, EPI_PLAIN only; tile.dma planes ahead.
+int row_dma_launch(const AxisStencil& st, const XmTile& tl, int dtype,
+                   const int n[3], const BC& bc, const void* in, void* out,
+                   long nin, hipStream_t stream, int* nblocks_out);
+bool row_dma_tile_ok(const XmTile& t, int R, int dtype, const int n[3]);
+
 }  // namespace gfd

@@ -58,6 +58,8 @@ struct XmTile {
     int nseg;    // row segments (threads along y)
     int py;      // rows per thread
     int xchunk;  // planes per block along x
+    int dma = 0; // full-row kernel: planes in flight by LDS DMA (k_xdma),
+                 // 0 for register prefetch (k_xrow)
 };
 // Launch one stencil pass over `nin` grids of shape n[3].
 // dtype: 0 = f64, 1 = c128, 2 = f32, 3 = c64 (row kernel and general
@@ -85,6 +87,8 @@ XmTile xr_default_tile(int R, int dtype, const int n[3], long nin);
 // Timing autotuner of the full-row kernel's tile (on by default; off also
 // forgets the tiles found).
 void xr_set_autotune(bool on);
+// Let the tuner try the LDS-DMA kernel (k_xdma) for plain real passes.
+void xr_set_dma(bool on);
 
 // Non-temporal hints of the full-row kernel: -1 automatic, else bit 0
 // output stores, bit 1 own-row loads.

@@ -151,9 +151,22 @@ struct XrKey {
     }
 };
 
+struct XrTuned {
+    XmTile row;           // fastest k_xrow tile (every epilogue)
+    XmTile dma;           // fastest k_xdma tile (plain passes), dma 0: none
+};
+
 static std::mutex g_tune_mu;
-static std::map<XrKey, XmTile> g_tuned;
+static std::map<XrKey, XrTuned> g_tuned;
 static bool g_autotune = true;
+static bool g_xdma = true;
+
+void xr_set_dma(bool on)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_xdma = on;
+    g_tuned.clear();
+}
 
 void xr_set_autotune(bool on)
 {
@@ -167,6 +180,14 @@ static int row_launch(const AxisStencil& st, const XmTile& tl, int dtype,
                       long nin, int epi, const EpiArgs& ea, hipStream_t stream,
                       int* nblocks_out)
 {
+    if (tl.dma > 0) {
+        if (epi != EPI_PLAIN) {
+            set_error("x-dma: plain passes only");
+            return ENOTSUP_;
+        }
+        return row_dma_launch(st, tl, dtype, n, bc, in, out, nin, stream,
+                              nblocks_out);
+    }
     switch (dtype) {
     case 0: return row_launch_f64(st, tl, n, bc, in, out, nin, epi, ea, stream, nblocks_out);
     case 1: return row_launch_c128(st, tl, n, bc, in, out, nin, epi, ea, stream, nblocks_out);
@@ -177,9 +198,9 @@ static int row_launch(const AxisStencil& st, const XmTile& tl, int dtype,
     return ENOTSUP_;
 }
 
-static XmTile xr_tune(const AxisStencil& st, int dtype, const int n[3],
-                      const BC& bc, const void* in, void* out, long nin,
-                      hipStream_t stream, const XmTile& heur)
+static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
+                       const BC& bc, const void* in, void* out, long nin,
+                       hipStream_t stream, const XmTile& heur, bool dma)
 {
     const int R = st.R;
     const int PZ = xr_pz(dtype);
@@ -216,25 +237,57 @@ static XmTile xr_tune(const AxisStencil& st, int dtype, const int n[3],
             t.xchunk = n[0];
             if (xr_tile_ok(t, R, dtype, n)) cand.push_back(t);
         }
+    if (dma && (dtype == 0 || dtype == 2)) {
+        // LDS-DMA ring: 1-2 planes in flight, x chunks of 16-64 planes or
+        // the whole grid (measured on 256 x 96^3 f64 nn=3: tall tiles and
+        // whole-grid chunks win, 8 x 48 threads x 4 rows, 2 planes ahead)
+        static const int dxcs[] = {16, 32, 64, 1 << 30};
+        for (int py = 1; py <= 4; py <<= 1)
+            for (int sg = 1; sg * TZT <= XR_MAXT; sg++) {
+                if (n[1] % (sg * py) != 0 || sg * TZT < 64) continue;
+                for (int A = 1; A <= 2; A++)
+                    for (int xc : dxcs) {
+                        if (xc >= n[0] && xc != dxcs[3]) continue;
+                        XmTile t;
+                        t.tz = n[2];
+                        t.nseg = sg;
+                        t.py = py;
+                        t.xchunk = std::min(xc, n[0]);
+                        t.dma = A;
+                        if (row_dma_tile_ok(t, R, dtype, n))
+                            cand.push_back(t);
+                    }
+            }
+    }
+    XrTuned res;
+    res.row = heur;
+    res.dma = heur;
+    res.dma.dma = 0;
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return heur;
+    if (hipEventCreate(&e0) != hipSuccess) return res;
     if (hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
-        return heur;
+        return res;
     }
     EpiArgs ea{};
     ea.c0 = st.c0;
-    XmTile best = heur;
-    float best_ms = 1e30f;
+    float best_ms = 1e30f, best_dma_ms = 1e30f;
     for (const XmTile& t : cand) {
+        // enough grids for at least 1024 blocks (4 per CU) where the batch
+        // has them: tiles with few blocks per grid would otherwise be
+        // timed on a partly idle device; compared per grid
+        const long per_grid = (long)(n[1] / (t.nseg * t.py)) *
+                              ((n[0] + t.xchunk - 1) / t.xchunk);
+        const long nt_c = std::min(
+            nin, std::max(nt_in, (1024 + per_grid - 1) / std::max(1L, per_grid)));
         // one untimed pass, then two timed
-        if (row_launch(st, t, dtype, n, bc, in, out, nt_in, EPI_PLAIN, ea,
+        if (row_launch(st, t, dtype, n, bc, in, out, nt_c, EPI_PLAIN, ea,
                        stream, nullptr) != OK)
             continue;
         (void)hipEventRecord(e0, stream);
         int st2 = OK;
         for (int r = 0; r < 2 && st2 == OK; r++)
-            st2 = row_launch(st, t, dtype, n, bc, in, out, nt_in, EPI_PLAIN,
+            st2 = row_launch(st, t, dtype, n, bc, in, out, nt_c, EPI_PLAIN,
                              ea, stream, nullptr);
         (void)hipEventRecord(e1, stream);
         if (hipEventSynchronize(e1) != hipSuccess || st2 != OK) {
@@ -243,32 +296,49 @@ static XmTile xr_tune(const AxisStencil& st, int dtype, const int n[3],
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
+        ms /= (float)nt_c;
+        if (t.dma == 0 && ms < best_ms) {
             best_ms = ms;
-            best = t;
+            res.row = t;
+        }
+        if (t.dma > 0 && ms < best_dma_ms) {
+            best_dma_ms = ms;
+            res.dma = t;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     set_error("");
-    return best;
+    // the DMA tile serves plain passes only where it is faster
+    if (res.dma.dma > 0 && !(best_dma_ms < best_ms)) res.dma.dma = 0;
+    return res;
 }
 
 static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
                            const BC& bc, const void* in, void* out, long nin,
                            int epi, const EpiArgs& ea, hipStream_t stream)
 {
+    const bool plain = epi == EPI_PLAIN;
+    auto choose = [&](const XrTuned& t) {
+        return plain && t.dma.dma > 0 ? t.dma : t.row;
+    };
     {
         std::lock_guard<std::mutex> lk(g_xr_mu);
-        if (g_xr_override) return g_xr_tile;
+        if (g_xr_override) {
+            XmTile t = g_xr_tile;
+            if (!plain) t.dma = 0;
+            return t;
+        }
     }
     const XmTile heur = xr_default_tile(st.R, dtype, n, nin);
     const XrKey key{st.R, dtype, n[0], n[1], n[2], nin};
+    bool dma;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         if (!g_autotune) return heur;
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) return it->second;
+        if (it != g_tuned.end()) return choose(it->second);
+        dma = g_xdma;
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
@@ -278,11 +348,11 @@ static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
     if (cs != hipStreamCaptureStatusNone || in == out ||
         (ea.aux && ea.aux == out) || (ea.pot && (const void*)ea.pot == out))
         return heur;
-    (void)epi;
-    const XmTile t = xr_tune(st, dtype, n, bc, in, out, nin, stream, heur);
+    const XrTuned t =
+        xr_tune(st, dtype, n, bc, in, out, nin, stream, heur, dma);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key] = t;
-    return t;
+    return choose(t);
 }
 
 int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
@@ -292,7 +362,8 @@ int launch_row_stencil(const AxisStencil& st, int dtype, const int n[3],
 {
     const XmTile tl =
         xr_pick_tile(st, dtype, n, bc, in, out, nin, epi, ea, stream);
-    if (!xr_tile_ok(tl, st.R, dtype, n)) {
+    if (tl.dma > 0 ? !row_dma_tile_ok(tl, st.R, dtype, n)
+                   : !xr_tile_ok(tl, st.R, dtype, n)) {
         set_error("x-row tile does not fit this grid");
         return ENOTSUP_;
     }
