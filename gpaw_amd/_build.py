@@ -17,7 +17,7 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = 'gfx950'
 
 # longest compiles first (the pool runs them in this order)
-SOURCES = ['rows_dma.hip', 'rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
+SOURCES = ['rows_dma.hip', 'rows_dma32.hip', 'rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
            'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
            'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip',
            'fftpoisson.hip']

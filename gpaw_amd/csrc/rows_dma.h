@@ -5,7 +5,7 @@
 // x planes per block), but planes reach LDS by `global_load_lds_dwordx4`:
 // the tile's TY + 2R rows of a plane are contiguous in the grid (up to the
 // periodic wrap in y), so each lane copies 16-byte chunks straight into a
-// ring of D = A + 2 plane slots without passing through VGPRs.  A planes
+// ring of A + 1 plane slots without passing through VGPRs.  A planes
 // are in flight while one is consumed, and the vector registers hold no
 // prefetched data, so the wave count per CU stays high while the bytes in
 // flight per CU grow with A.
@@ -20,12 +20,15 @@
 //    valid dummy address) and overwrite their chunks with zeros after it
 //    has landed, so every wave issues the same number of DMAs per plane
 //    and the vmcnt bound of each wait is exact.
-//  * synchronisation per plane: issue the DMA of plane i + A into the slot
-//    last read at step i - 2, wait until this wave's DMA of plane i has
-//    landed (vmcnt = DMAs issued after it), write zero chunks, one
-//    barrier.  Vector memory operations complete in issue order, so later
-//    stores only make the wait stricter.
+//  * synchronisation per plane: wait until this wave's DMA of plane i has
+//    landed (vmcnt = DMAs issued after it), write zero chunks, one barrier,
+//    then issue the DMA of plane i + A into the slot read at step i - 1
+//    (every wave is past that step), so the ring needs A + 1 slots.
+//    Vector memory operations complete in issue order, so stores issued
+//    after a DMA only make the wait stricter.
 #pragma once
+
+#include <algorithm>
 
 #include "rows_kernel.h"
 
@@ -41,10 +44,8 @@ struct XdArgs {
     long ng;
     int tiles_y, chunks_x, xchunk;
     int nblocks;
-    int ahead;          // planes in flight (A); ring of A + 2 slots
-    int kd;             // DMA chunks per thread per plane
+    int ahead;          // planes in flight (A); ring of A + 1 slots
     int slot_chunks;    // kd * threads: 16-byte chunks per slot
-    int hasx, hasy, hasz;
     double c0;
     double cp[3][MAXR_FAST];
     double cm[3][MAXR_FAST];
@@ -66,26 +67,14 @@ __device__ inline void xd_dma16(const void* g, unsigned lds)
 
 #define XD_VMCNT(n) ((((n) & 0xF) | (((n) >> 4) << 14)) | 0x70 | 0xF00)
 
-// s_waitcnt vmcnt(n) lgkmcnt(0) for a runtime n (the immediate must be a
-// constant: one branch per plane)
-__device__ inline void xd_wait(int n)
-{
-    switch (n) {
-#define XD_W(k) case k: __builtin_amdgcn_s_waitcnt(XD_VMCNT(k) & ~0xF00); break;
-        XD_W(0) XD_W(1) XD_W(2) XD_W(3) XD_W(4) XD_W(5) XD_W(6) XD_W(7)
-        XD_W(8) XD_W(9) XD_W(10) XD_W(11) XD_W(12) XD_W(13) XD_W(14)
-        XD_W(15) XD_W(16) XD_W(17) XD_W(18) XD_W(19) XD_W(20) XD_W(21)
-        XD_W(22) XD_W(23) XD_W(24) XD_W(25) XD_W(26) XD_W(27) XD_W(28)
-        XD_W(29) XD_W(30) XD_W(31) XD_W(32)
-#undef XD_W
-    default: __builtin_amdgcn_s_waitcnt(0); break;
-    }
-}
-
-template <typename T, int R, int PY, int PZ>
+// H = ceil(2R / nseg): halo chunks per thread per plane, so that each
+// thread issues KD = PY + H DMAs per plane (a compile-time count: the vmcnt
+// bounds are immediates and the DMA loop has no guards).
+template <typename T, int R, int PY, int PZ, int H>
 __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
 {
     typedef Chunk<T, PZ> C;
+    constexpr int KD = PY + H;
     constexpr int RC = (R + PZ - 1) / PZ;
     constexpr int NW = 2 * RC + 1;
     constexpr int RP = RC * PZ;
@@ -111,21 +100,20 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
     const int xe = min(xs + a.xchunk, n0);
     const int NL = xe - xs + 2 * R;   // landings: planes xs-R .. xe+R-1
     const int A = a.ahead;
-    const int D = A + 2;
-    const int KD = a.kd;
+    const int D = A + 1;
     const int SLOT = a.slot_chunks * 16;   // bytes
     const T* __restrict__ gin = a.in + b * a.ng;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wave_base = wave * 64;       // first chunk of the wave per k
 
     // DMA chunks of this thread: c = k * NT + tid, row c / CPR of the slot
-    int doff[XD_KDMAX];
-    bool dzero[XD_KDMAX];
+    int doff[KD];
+    bool dzero[KD];
 #pragma unroll
-    for (int k = 0; k < XD_KDMAX; k++) {
+    for (int k = 0; k < KD; k++) {
         doff[k] = 0;
         dzero[k] = false;
-        if (k < KD) {
+        {
             const int c = k * NT + tid;
             const int r = c / CPR, col = c - (c / CPR) * CPR;
             if (r < TH) {
@@ -141,8 +129,8 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
         }
     }
 
-    auto slot_ptr = [&](int i) -> unsigned char* {
-        return smem_raw + (i % D) * SLOT;
+    auto slot_ptr = [&](int si) -> unsigned char* {
+        return smem_raw + si * SLOT;
     };
     // plane of landing i: offset, or -1 for a zero plane
     auto plane_off = [&](int i) -> long {
@@ -153,22 +141,24 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
         }
         return (long)x * plane;
     };
-    auto issue = [&](int i) {
+    auto issue = [&](int i, int si) {
         const long po = plane_off(i);
         const T* src = gin + (po < 0 ? 0 : po);
-        unsigned char* s = slot_ptr(i);
+        unsigned char* s = slot_ptr(si);
 #pragma unroll
-        for (int k = 0; k < XD_KDMAX; k++)
-            if (k < KD)
+        for (int k = 0; k < KD; k++)
                 xd_dma16(src + doff[k],
                          (unsigned)(size_t)(xd_lds_t*)(s + (k * NT + wave_base) * 16));
     };
-    auto zero_fill = [&](int i) {
+    // zero rows exist only in the edge tiles of a zero-boundary y axis
+    const bool zrows = !a.periodic[1] && (y0 < R || y0 + TY + R > n1);
+    auto zero_fill = [&](int i, int si) {
         const bool zp = plane_off(i) < 0;
-        unsigned char* s = slot_ptr(i);
+        if (!zp && !zrows) return;
+        unsigned char* s = slot_ptr(si);
 #pragma unroll
-        for (int k = 0; k < XD_KDMAX; k++)
-            if (k < KD && (zp || dzero[k]))
+        for (int k = 0; k < KD; k++)
+            if (zp || dzero[k])
                 *reinterpret_cast<C*>(s + (k * NT + tid) * 16) =
                     zero_chunk<T, PZ>();
     };
@@ -209,31 +199,39 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
 #pragma unroll
         for (int j = 0; j < PY; j++) acc[q][j] = zero_chunk<T, PZ>();
 
-    for (int i = 0; i < A && i < NL; i++) issue(i);
+    for (int i = 0; i < A && i < NL; i++) issue(i, i);
+    int s_cur = 0;        // slot of landing i
+    int s_nxt = A;        // slot of landing i + A
 
     for (int ig = 0; ig < NL; ig += QU) {
 #pragma unroll
         for (int u = 0; u < QU; u++) {
             const int i = ig + u;
             if (i < NL) {
-                if (i + A < NL) issue(i + A);
-                // DMAs issued after landing i: planes i+1 .. min(i+A, NL-1)
-                xd_wait(KD * min(A, NL - 1 - i));
+                // DMAs issued after landing i: planes i+1 .. min(i+A-1,
+                // NL-1), KD each (A is 1 or 2)
+                if (A == 1 || i + 1 >= NL)
+                    __builtin_amdgcn_s_waitcnt(XD_VMCNT(0) & ~0xF00);
+                else
+                    __builtin_amdgcn_s_waitcnt(XD_VMCNT(KD) & ~0xF00);
                 asm volatile("" ::: "memory");
-                zero_fill(i);
+                zero_fill(i, s_cur);
                 __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
                 // the waits are explicit; keep the compiler from moving
                 // LDS accesses across the barrier
                 asm volatile("" ::: "memory");
                 __builtin_amdgcn_s_barrier();
                 asm volatile("" ::: "memory");
+                if (i + A < NL) issue(i + A, s_nxt);
 
-                const T* s = reinterpret_cast<const T*>(slot_ptr(i));
+                const T* s = reinterpret_cast<const T*>(slot_ptr(s_cur));
+                s_cur = s_cur + 1 == D ? 0 : s_cur + 1;
+                s_nxt = s_nxt + 1 == D ? 0 : s_nxt + 1;
                 const int p = xs - R + i;
                 const bool own = p >= xs && p < xe;
                 // column chunks of rows lrow0 - R .. lrow0 + PY - 1 + R
                 C col[PY + 2 * R];
-                if (own && a.hasy) {
+                if (own) {
 #pragma unroll
                     for (int r = 0; r < PY + 2 * R; r++)
                         col[r] = *reinterpret_cast<const C*>(
@@ -247,8 +245,9 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
 #pragma unroll
                 for (int j = 0; j < PY; j++) {
                     const C v = col[R + j];
-                    // x terms into outputs p - d (d = -R..R, d != 0)
-                    if (a.hasx) {
+                    // x terms into outputs p - d (d = -R..R, d != 0); axes
+                    // a stencil lacks have zero coefficients
+                    {
 #pragma unroll
                         for (int k = 1; k <= R; k++) {
                             // plane p is x + k for output x = p - k
@@ -266,7 +265,7 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
 #pragma unroll
                         for (int q = 0; q < PZ; q++)
                             me.v[q] = fmac(me.v[q], c0, v.v[q]);
-                        if (a.hasy) {
+                        {
 #pragma unroll
                             for (int k = 1; k <= R; k++)
 #pragma unroll
@@ -277,7 +276,7 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
                                                    col[R + j - k].v[q]);
                                 }
                         }
-                        if (a.hasz) {
+                        {
                             const int lrow = (lrow0 + j) * n2;
                             T wz[NW * PZ];
 #pragma unroll
@@ -332,20 +331,20 @@ inline size_t xd_lds_bytes(const XmTile& t, int R, int es, const int n[3],
                            int* kd_out = nullptr)
 {
     const int PZ = 16 / es;
-    if (t.dma < 1 || t.dma > 4 || n[2] % PZ != 0) return 0;
+    if (t.dma < 1 || t.dma > 2 || n[2] % PZ != 0 || 2 * R > 2 * t.nseg) return 0;
     const int CPR = n[2] / PZ;
     const int NT = CPR * t.nseg;
     const int TH = t.nseg * t.py + 2 * R;
     if (NT > XR_MAXT || NT < 64) return 0;
     const int kd = (TH * CPR + NT - 1) / NT;
-    if (kd > XD_KDMAX || kd * t.dma > 32) return 0;
-    const size_t bytes = (size_t)(t.dma + 2) * kd * NT * 16;
+    if (kd > XD_KDMAX) return 0;
+    const size_t bytes = (size_t)(t.dma + 1) * kd * NT * 16;
     if (bytes > 160 * 1024) return 0;
     if (kd_out) *kd_out = kd;
     return bytes;
 }
 
-template <typename T, int R, int PY>
+template <typename T, int R, int PY, int H>
 static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
                        const int n[3], const BC& bc, const void* in,
                        void* out, long nin, hipStream_t stream,
@@ -354,6 +353,11 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
     constexpr int PZ = PzOf<T>::value;
     int kd = 0;
     const size_t shm = xd_lds_bytes(tl, R, (int)sizeof(T), n, &kd);
+    if (shm != 0 && kd != PY + H) {
+        set_error("x-dma: %d DMA chunks per thread, kernel built for %d", kd,
+                  PY + H);
+        return ENOTSUP_;
+    }
     if (shm == 0 || n[1] % (tl.nseg * PY) != 0 || R > n[0] || R > n[1] ||
         n[2] < ((R + PZ - 1) / PZ) * PZ) {
         set_error("x-dma: tile does not fit this grid");
@@ -378,12 +382,8 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
     }
     a.nblocks = (int)nb;
     a.ahead = tl.dma;
-    a.kd = kd;
     const int NT = (n[2] / PZ) * tl.nseg;
     a.slot_chunks = kd * NT;
-    a.hasx = st.has[0];
-    a.hasy = st.has[1];
-    a.hasz = st.has[2];
     a.c0 = st.c0;
     for (int ax = 0; ax < 3; ax++) {
         a.periodic[ax] = bc.periodic[ax];
@@ -396,7 +396,7 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
     static size_t attr_bytes = 64 * 1024;
     if (shm > attr_bytes) {
         hipError_t e = hipFuncSetAttribute(
-            (const void*)k_xdma<T, R, PY, PZ>,
+            (const void*)k_xdma<T, R, PY, PZ, H>,
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -407,7 +407,7 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
         attr_bytes = shm;
     }
     (void)hipGetLastError();
-    hipLaunchKernelGGL((k_xdma<T, R, PY, PZ>), dim3((unsigned)nb),
+    hipLaunchKernelGGL((k_xdma<T, R, PY, PZ, H>), dim3((unsigned)nb),
                        dim3(n[2] / PZ, tl.nseg), shm, stream, a);
     GFD_LAUNCH_CHECK();
     if (nblocks_out) *nblocks_out = (int)nb;
@@ -419,14 +419,19 @@ static int launch_xd(const AxisStencil& st, const XmTile& tl, const int n[3],
                      const BC& bc, const void* in, void* out, long nin,
                      hipStream_t stream, int* nbo)
 {
-#define XD_CASE(RR, PP)                                                      \
-    if (st.R == RR && tl.py == PP)                                           \
-        return launch_xd_t<T, RR, PP>(st, tl, n, bc, in, out, nin, stream,   \
-                                      nbo);
-    XD_CASE(1, 1) XD_CASE(1, 2) XD_CASE(1, 4)
-    XD_CASE(2, 1) XD_CASE(2, 2) XD_CASE(2, 4)
-    XD_CASE(3, 1) XD_CASE(3, 2) XD_CASE(3, 4)
-    XD_CASE(4, 1) XD_CASE(4, 2) XD_CASE(4, 4)
+    const int h = (2 * st.R + tl.nseg - 1) / std::max(1, tl.nseg);
+#define XD_CASE(RR, PP, HH)                                                  \
+    if (st.R == RR && tl.py == PP && h == HH)                                \
+        return launch_xd_t<T, RR, PP, HH>(st, tl, n, bc, in, out, nin,       \
+                                          stream, nbo);
+    XD_CASE(1, 1, 1) XD_CASE(1, 2, 1) XD_CASE(1, 4, 1)
+    XD_CASE(1, 1, 2) XD_CASE(1, 2, 2) XD_CASE(1, 4, 2)
+    XD_CASE(2, 1, 1) XD_CASE(2, 2, 1) XD_CASE(2, 4, 1)
+    XD_CASE(2, 1, 2) XD_CASE(2, 2, 2) XD_CASE(2, 4, 2)
+    XD_CASE(3, 1, 1) XD_CASE(3, 2, 1) XD_CASE(3, 4, 1)
+    XD_CASE(3, 1, 2) XD_CASE(3, 2, 2) XD_CASE(3, 4, 2)
+    XD_CASE(4, 1, 1) XD_CASE(4, 2, 1) XD_CASE(4, 4, 1)
+    XD_CASE(4, 1, 2) XD_CASE(4, 2, 2) XD_CASE(4, 4, 2)
 #undef XD_CASE
     set_error("x-dma: unsupported radius %d / rows %d", st.R, tl.py);
     return ENOTSUP_;

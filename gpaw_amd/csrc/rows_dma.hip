@@ -1,7 +1,12 @@
-// k_xdma instantiations: real data, plain stencil pass.
+// k_xdma instantiations for double data (float: rows_dma32.hip).
 #include "rows_dma.h"
 
 namespace gfd {
+
+int row_dma_launch_f32(const AxisStencil& st, const XmTile& tl,
+                       const int n[3], const BC& bc, const void* in,
+                       void* out, long nin, hipStream_t stream,
+                       int* nblocks_out);
 
 int row_dma_launch(const AxisStencil& st, const XmTile& tl, int dtype,
                    const int n[3], const BC& bc, const void* in, void* out,
@@ -11,8 +16,8 @@ int row_dma_launch(const AxisStencil& st, const XmTile& tl, int dtype,
         return launch_xd<double>(st, tl, n, bc, in, out, nin, stream,
                                  nblocks_out);
     if (dtype == 2)
-        return launch_xd<float>(st, tl, n, bc, in, out, nin, stream,
-                                nblocks_out);
+        return row_dma_launch_f32(st, tl, n, bc, in, out, nin, stream,
+                                  nblocks_out);
     set_error("x-dma: real data only");
     return ENOTSUP_;
 }
@@ -21,9 +26,10 @@ bool row_dma_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
 {
     if (dtype != 0 && dtype != 2) return false;
     const int es = dtype == 0 ? 8 : 4;
+    const int pz = 16 / es;
     return xd_lds_bytes(t, R, es, n) > 0 && n[1] % (t.nseg * t.py) == 0 &&
            (t.py == 1 || t.py == 2 || t.py == 4) && R <= n[0] && R <= n[1] &&
-           t.xchunk >= 1 && n[2] >= ((R + 16 / es - 1) / (16 / es)) * (16 / es);
+           t.xchunk >= 1 && n[2] >= ((R + pz - 1) / pz) * pz;
 }
 
 }  // namespace gfd
