@@ -51,7 +51,23 @@ struct XdArgs {
     double cm[3][MAXR_FAST];
     int periodic[3];
     int nt;
+    int order;          // block order: 0 XCD-contiguous tiles, 1 linear,
+                        // 2 bands interleaved over the 8 XCDs
 };
+
+// Logical tile of hardware block h (blocks reach the 8 XCDs round robin).
+// 0: consecutive tiles share an XCD (xcd_remap); 1: identity, so the
+// blocks in flight cover one compact address window; 2: all tiles of band
+// b on XCD b % 8, the XCDs on neighbouring bands (needs nin % 8 == 0).
+__device__ inline int xd_block(int h, int nb, int order, int per_band)
+{
+    if (order == 1) return h;
+    if (order == 2) {
+        const int x = h & 7, k = h >> 3;
+        return (x + 8 * (k / per_band)) * per_band + k % per_band;
+    }
+    return xcd_remap(h, nb);
+}
 
 typedef __attribute__((address_space(3))) unsigned char xd_lds_t;
 
@@ -81,7 +97,8 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
     constexpr int QU = 2 * R + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
 
-    const int L = xcd_remap(blockIdx.x, a.nblocks);
+    const int L = xd_block(blockIdx.x, a.nblocks, a.order,
+                           a.tiles_y * a.chunks_x);
     const int ty = L % a.tiles_y;
     const int cx = (L / a.tiles_y) % a.chunks_x;
     const long b = L / (a.tiles_y * a.chunks_x);
@@ -353,6 +370,11 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
     constexpr int PZ = PzOf<T>::value;
     int kd = 0;
     const size_t shm = xd_lds_bytes(tl, R, (int)sizeof(T), n, &kd);
+    if (tl.order < 0 || tl.order > 2 || (tl.order == 2 && nin % 8 != 0)) {
+        set_error("x-dma: block order %d not usable for %ld grids", tl.order,
+                  nin);
+        return ENOTSUP_;
+    }
     if (shm != 0 && kd != PY + H) {
         set_error("x-dma: %d DMA chunks per thread, kernel built for %d", kd,
                   PY + H);
@@ -382,6 +404,7 @@ static int launch_xd_t(const AxisStencil& st, const XmTile& tl,
     }
     a.nblocks = (int)nb;
     a.ahead = tl.dma;
+    a.order = tl.order;
     const int NT = (n[2] / PZ) * tl.nseg;
     a.slot_chunks = kd * NT;
     a.c0 = st.c0;

@@ -241,22 +241,24 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
         // LDS-DMA ring: 1-2 planes in flight, x chunks of 16-64 planes or
         // the whole grid (measured on 256 x 96^3 f64 nn=3: tall tiles and
         // whole-grid chunks win, 8 x 48 threads x 4 rows, 2 planes ahead)
-        static const int dxcs[] = {16, 32, 64, 1 << 30};
+        static const int dxcs[] = {16, 32, 1 << 30};
         for (int py = 1; py <= 4; py <<= 1)
             for (int sg = 1; sg * TZT <= XR_MAXT; sg++) {
                 if (n[1] % (sg * py) != 0 || sg * TZT < 64) continue;
                 for (int A = 1; A <= 2; A++)
-                    for (int xc : dxcs) {
-                        if (xc >= n[0] && xc != dxcs[3]) continue;
-                        XmTile t;
-                        t.tz = n[2];
-                        t.nseg = sg;
-                        t.py = py;
-                        t.xchunk = std::min(xc, n[0]);
-                        t.dma = A;
-                        if (row_dma_tile_ok(t, R, dtype, n))
-                            cand.push_back(t);
-                    }
+                    for (int xc : dxcs)
+                        for (int order = 0; order <= 2; order += 2) {
+                            if (xc >= n[0] && xc != dxcs[2]) continue;
+                            XmTile t;
+                            t.tz = n[2];
+                            t.nseg = sg;
+                            t.py = py;
+                            t.xchunk = std::min(xc, n[0]);
+                            t.dma = A;
+                            t.order = order;
+                            if (row_dma_tile_ok(t, R, dtype, n))
+                                cand.push_back(t);
+                        }
             }
     }
     XrTuned res;

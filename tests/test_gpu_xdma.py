@@ -119,3 +119,32 @@ def test_xdma_gradient_and_tuner():
         lib.gfd_set_tuning(b'xdma', 0)
     for band in (0, 63):
         assert relerr(host(out[band]), ref_apply(op, host(b[band]))) < TOL
+
+
+@pytest.mark.parametrize('order', [0, 1, 2])
+def test_xdma_block_orders(order):
+    """The three block orders of the DMA kernel (XCD-contiguous tiles,
+    linear, bands interleaved over XCDs; the last needs a multiple of 8
+    grids) on 8 grids with short x chunks."""
+    lib = _lib.load()
+    rng = np.random.default_rng(80 + order)
+    gd = GridDescriptor((16, 25, 32), (3.0, 2.0, 4.0), (True, False, True))
+    n = tuple(int(x) for x in gd.n_c)
+    op = Laplace(gd, -0.5, 3)
+    a = rand(rng, (8,) + n)
+    ref = ref_apply(op, a)
+    nseg, py = next((s, p) for s in range(3, 9) for p in (1, 2)
+                    if tile_ok(n, 3, s, p, 2))
+    try:
+        _lib.check(lib.gfd_set_tuning(
+            b'xr_tile', tile_code(n[2], nseg, py, 5, 2) | (order << 52)),
+            'tuning')
+        with kernel_path('row'):
+            say('xdma order', order, nseg, py)
+            d = dev(a)
+            out = d.new_empty(d.shape)
+            op.apply(d, out)
+            got = host(out)
+    finally:
+        lib.gfd_set_tuning(b'xr_tile', 0)
+    assert relerr(got, ref) < TOL
