@@ -140,8 +140,11 @@ long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
 struct XrKey {
     int R, dtype, n0, n1, n2;
     long nin;
+    int axes;  // axes the stencil has (bit per axis): a one-axis stencil
+               // runs without LDS in k_xrow and tunes differently
     bool operator<(const XrKey& o) const
     {
+        if (axes != o.axes) return axes < o.axes;
         if (R != o.R) return R < o.R;
         if (dtype != o.dtype) return dtype < o.dtype;
         if (n0 != o.n0) return n0 < o.n0;
@@ -274,14 +277,19 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
     EpiArgs ea{};
     ea.c0 = st.c0;
     float best_ms = 1e30f, best_dma_ms = 1e30f;
+    // every candidate on the same grids, enough for at least 1024 blocks
+    // (4 per CU) of the candidate with the fewest blocks per grid where the
+    // batch has them: such tiles would otherwise be timed on a partly idle
+    // device
+    long min_per_grid = 1L << 40;
+    for (const XmTile& t : cand)
+        min_per_grid = std::min(min_per_grid,
+                                (long)(n[1] / (t.nseg * t.py)) *
+                                    ((n[0] + t.xchunk - 1) / t.xchunk));
+    const long nt_c = std::min(
+        nin, std::max(nt_in, (1024 + min_per_grid - 1) /
+                                 std::max(1L, min_per_grid)));
     for (const XmTile& t : cand) {
-        // enough grids for at least 1024 blocks (4 per CU) where the batch
-        // has them: tiles with few blocks per grid would otherwise be
-        // timed on a partly idle device; compared per grid
-        const long per_grid = (long)(n[1] / (t.nseg * t.py)) *
-                              ((n[0] + t.xchunk - 1) / t.xchunk);
-        const long nt_c = std::min(
-            nin, std::max(nt_in, (1024 + per_grid - 1) / std::max(1L, per_grid)));
         // one untimed pass, then two timed
         if (row_launch(st, t, dtype, n, bc, in, out, nt_c, EPI_PLAIN, ea,
                        stream, nullptr) != OK)
@@ -298,7 +306,6 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)nt_c;
         if (t.dma == 0 && ms < best_ms) {
             best_ms = ms;
             res.row = t;
@@ -333,7 +340,8 @@ static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
         }
     }
     const XmTile heur = xr_default_tile(st.R, dtype, n, nin);
-    const XrKey key{st.R, dtype, n[0], n[1], n[2], nin};
+    const XrKey key{st.R, dtype, n[0], n[1], n[2], nin,
+                    st.has[0] | st.has[1] << 1 | st.has[2] << 2};
     bool dma;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
