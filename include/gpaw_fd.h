@@ -329,9 +329,13 @@ int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
  * full-row x-march; a family that cannot take a grid falls back to the
  * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
  * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
- * the tile of the tiled / full-row x-march kernel (value 0 clears);
+ * the tile of the tiled / full-row x-march kernel (value 0 clears); for
+ * "xr_tile", dma << 48 (1-2 planes in flight) selects the LDS-DMA variant
+ * for plain passes on real data and order << 52 its block order (0
+ * XCD-contiguous tiles, 1 linear, 2 bands interleaved over the XCDs);
  * "autotune" (0/1, default 1) times candidate full-row tiles on the first
- * launch of each shape (off: heuristic tiles); "xr_nt" (-1 automatic, else
+ * launch of each shape (off: heuristic tiles); "xdma" (0/1, default 1)
+ * lets it time the LDS-DMA variant too; "xr_nt" (-1 automatic, else
  * bit 0 non-temporal stores, bit 1 non-temporal loads);
  * "coarse_lds" (0/1, default 1) runs the coarse levels of the Poisson
  * V-cycles that fit in one workgroup's LDS as a single launch;
