@@ -20,7 +20,7 @@ ARCH = 'gfx950'
 SOURCES = ['rows_dma.hip', 'rows_dma32.hip', 'rows_f64.hip', 'rows_c128.hip', 'rows_f32.hip', 'rows_c64.hip',
            'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
            'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip',
-           'fftpoisson.hip']
+           'fftpoisson.hip', 'wstencil.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',

@@ -76,6 +76,12 @@ SIGNATURES = {
                                  ctypes.POINTER(_vp)]),
     'gfd_fft_poisson_free': (None, [_vp]),
     'gfd_fft_poisson_solve': (_i, [_vp, _vp, _vp, _vp]),
+    'gfd_woperator_new': (_i, [_i, _vp, _dp, _lp, _lp, _i, _lp, _i, _vp,
+                               ctypes.POINTER(_vp)]),
+    'gfd_woperator_free': (None, [_vp]),
+    'gfd_woperator_apply': (_i, [_vp, _vp, _vp, _l, _dp, _vp]),
+    'gfd_woperator_relax': (_i, [_vp, _i, _vp, _vp, _i, _d, _vp]),
+    'gfd_woperator_get_diagonal_element': (_i, [_vp, _dp, _vp]),
     'gfd_h_residual': (_i, [_vp, _vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_dots': (_i, [_i, _vp, _vp, _i, _l, _l, _d, _vp, _vp]),
     'gfd_rmm_lambda': (_i, [_i, _vp, _vp, _i, _l, _l, _vp, _vp]),

@@ -1,0 +1,378 @@
+// Weighted FD operators (WeightedFDOperator, the dielectric Poisson
+// operator of the solvation code): out(x) = sum_w weight_w(x) *
+// sum_p c_{w,p} in(x + o_{w,p}), and its weighted Jacobi relaxation
+// (c/woperators.cpp, c/bmgs/wfd.cpp, c/bmgs/wrelax.cpp).
+//
+// One thread per grid point; the neighbours of every stencil come through
+// L1/L2 (the stencils of one operator share their offsets, so each
+// neighbour is fetched from memory once and re-read from cache), the
+// weights are read once per point.  Wrapping and Bloch phases as in the
+// other stencil kernels (wrap_full); no padded copies.
+#include "../../include/gpaw_fd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "handles.h"
+#include "stencil_dev.h"
+
+namespace gfd {
+
+struct WArgs {
+    int nw;
+    int n[3];
+    long ng;
+    const int* start;          // [nw + 1]: first point of each stencil
+    const double* coef;        // [points]
+    const int* off;            // [points][3]
+    const double* const* wts;  // [nw] device pointers, ng doubles each
+    BC bc;
+};
+
+template <typename T>
+__device__ inline T wneighbour(const WArgs& a, const T* g, int i0, int i1,
+                               int i2, int c, bool& ok)
+{
+    int j0 = i0 + a.off[3 * c], j1 = i1 + a.off[3 * c + 1],
+        j2 = i2 + a.off[3 * c + 2];
+    double2 f = make_double2(1.0, 0.0);
+    bool ph = false;
+    ok = wrap_full(j0, a.n[0], a.bc.periodic[0], a.bc.phase[0][0],
+                   a.bc.phase[0][1], f, ph) &&
+         wrap_full(j1, a.n[1], a.bc.periodic[1], a.bc.phase[1][0],
+                   a.bc.phase[1][1], f, ph) &&
+         wrap_full(j2, a.n[2], a.bc.periodic[2], a.bc.phase[2][0],
+                   a.bc.phase[2][1], f, ph);
+    if (!ok) return zero_of<T>();
+    const T v = g[((long)j0 * a.n[1] + j1) * a.n[2] + j2];
+    return scale_by<T>(v, f, ph);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_wapply(WArgs a, const T* __restrict__ in, T* __restrict__ out, long nin)
+{
+    typedef typename ScalarOf<T>::type S;
+    const long total = a.ng * nin;
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < total;
+         p += (long)gridDim.x * 256) {
+        const long b = p / a.ng, idx = p - b * a.ng;
+        const int i2 = (int)(idx % a.n[2]);
+        const long q = idx / a.n[2];
+        const int i1 = (int)(q % a.n[1]), i0 = (int)(q / a.n[1]);
+        const T* g = in + b * a.ng;
+        T x = zero_of<T>();
+        for (int w = 0; w < a.nw; w++) {
+            T t = zero_of<T>();
+            for (int c = a.start[w]; c < a.start[w + 1]; c++) {
+                bool ok;
+                const T v = wneighbour(a, g, i0, i1, i2, c, ok);
+                if (ok) t = fmac(t, (S)a.coef[c], v);
+            }
+            x = fmac(x, (S)a.wts[w][idx], t);
+        }
+        out[p] = x;
+    }
+}
+
+// Weighted Jacobi sweep: nxt = (1 - w) cur + w (src - x) / coef with
+// x = sum_w weight_w sum_{p >= 1} c_{w,p} cur(x + o), coef = sum_w weight_w
+// c_{w,0} (the first point of every stencil is the centre)
+__global__ void __launch_bounds__(256)
+k_wrelax(WArgs a, const double* __restrict__ cur, double* __restrict__ nxt,
+         const double* __restrict__ src, double w)
+{
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < a.ng;
+         idx += (long)gridDim.x * 256) {
+        const int i2 = (int)(idx % a.n[2]);
+        const long q = idx / a.n[2];
+        const int i1 = (int)(q % a.n[1]), i0 = (int)(q / a.n[1]);
+        double x = 0.0, coef = 0.0;
+        for (int v = 0; v < a.nw; v++) {
+            const double wt = a.wts[v][idx];
+            double t = 0.0;
+            for (int c = a.start[v] + 1; c < a.start[v + 1]; c++) {
+                bool ok;
+                const double nb = wneighbour(a, cur, i0, i1, i2, c, ok);
+                if (ok) t = fma(a.coef[c], nb, t);
+            }
+            x = fma(wt, t, x);
+            coef = fma(wt, a.coef[a.start[v]], coef);
+        }
+        nxt[idx] = (1.0 - w) * cur[idx] + w * (src[idx] - x) / coef;
+    }
+}
+
+// per-block max over points of |sum_w weight_w c_{w,0}|
+__global__ void __launch_bounds__(256)
+k_wdiag(WArgs a, double* __restrict__ partial)
+{
+    double m = 0.0;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < a.ng;
+         idx += (long)gridDim.x * 256) {
+        double coef = 0.0;
+        for (int v = 0; v < a.nw; v++)
+            coef = fma(a.wts[v][idx], a.coef[a.start[v]], coef);
+        m = fmax(m, fabs(coef));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+static unsigned wgrid(long n)
+{
+    long b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace gfd
+
+using namespace gfd;
+
+struct gfd_woperator {
+    int nw = 0;
+    int n[3] = {0, 0, 0};
+    int periodic[3] = {0, 0, 0};
+    int real = 1;
+    int* d_start = nullptr;
+    double* d_coef = nullptr;
+    int* d_off = nullptr;
+    const double** d_wts = nullptr;
+    DevBuf scratch;
+    DevBuf partial;
+    long ng() const { return (long)n[0] * n[1] * n[2]; }
+    WArgs args(const double* phases) const
+    {
+        WArgs a;
+        a.nw = nw;
+        for (int c = 0; c < 3; c++) a.n[c] = n[c];
+        a.ng = ng();
+        a.start = d_start;
+        a.coef = d_coef;
+        a.off = d_off;
+        a.wts = d_wts;
+        for (int c = 0; c < 3; c++) {
+            a.bc.periodic[c] = periodic[c];
+            for (int d = 0; d < 2; d++)
+                a.bc.phase[c][d] =
+                    phases ? make_double2(phases[4 * c + 2 * d],
+                                          phases[4 * c + 2 * d + 1])
+                           : make_double2(1.0, 0.0);
+        }
+        return a;
+    }
+};
+
+extern "C" void gfd_woperator_free(gfd_woperator* op)
+{
+    if (!op) return;
+    if (op->d_start) (void)hipFree(op->d_start);
+    if (op->d_coef) (void)hipFree(op->d_coef);
+    if (op->d_off) (void)hipFree(op->d_off);
+    if (op->d_wts) (void)hipFree((void*)op->d_wts);
+    op->scratch.release();
+    op->partial.release();
+    delete op;
+}
+
+extern "C" int gfd_woperator_new(int nweights, const int* ncoefs,
+                                 const double* coefs, const long* offsets,
+                                 const long size[3], int range,
+                                 const long neighbors[6], int real,
+                                 const double* const* weights,
+                                 gfd_woperator** out)
+{
+    if (!out || nweights <= 0 || !ncoefs || !coefs || !offsets || !size ||
+        !neighbors || !weights || range < 0) {
+        set_error("gfd_woperator_new: bad arguments");
+        return EINVAL_;
+    }
+    *out = nullptr;
+    auto* op = new gfd_woperator;
+    op->nw = nweights;
+    op->real = real ? 1 : 0;
+    for (int c = 0; c < 3; c++) {
+        if (size[c] <= 0 || size[c] > (1L << 20)) {
+            gfd_woperator_free(op);
+            set_error("gfd_woperator_new: bad grid size");
+            return EINVAL_;
+        }
+        op->n[c] = (int)size[c];
+        const bool lo = neighbors[2 * c] >= 0, hi = neighbors[2 * c + 1] >= 0;
+        if (lo != hi) {
+            gfd_woperator_free(op);
+            set_error("axis %d: mixed periodic / zero sides are not "
+                      "supported on a single domain", c);
+            return ENOTSUP_;
+        }
+        op->periodic[c] = lo ? 1 : 0;
+    }
+    // decode the padded-stride offsets (as gfd_operator_new); the first
+    // point of every stencil must be the centre
+    const long M1 = size[1] + 2 * range, M2 = size[2] + 2 * range;
+    const long shift = range * (M1 * M2 + M2 + 1);
+    std::vector<int> start(nweights + 1, 0), off;
+    long total = 0;
+    for (int w = 0; w < nweights; w++) {
+        if (ncoefs[w] <= 0 || ncoefs[w] > MAXP) {
+            gfd_woperator_free(op);
+            set_error("gfd_woperator_new: bad point count");
+            return EINVAL_;
+        }
+        start[w] = (int)total;
+        total += ncoefs[w];
+    }
+    start[nweights] = (int)total;
+    off.resize(3 * total);
+    for (long p = 0; p < total; p++) {
+        const long t = offsets[p] + shift;
+        const long o2 = t % M2 - range;
+        const long o1 = (t / M2) % M1 - range;
+        const long o0 = t / (M1 * M2) - range;
+        if (t < 0 || o0 < -range || o0 > range) {
+            gfd_woperator_free(op);
+            set_error("gfd_woperator_new: offset %ld outside range %d",
+                      offsets[p], range);
+            return EINVAL_;
+        }
+        const long oc[3] = {o0, o1, o2};
+        for (int c = 0; c < 3; c++) {
+            if (oc[c] > size[c] || -oc[c] > size[c]) {
+                gfd_woperator_free(op);
+                set_error("Stencil longer than domain");
+                return EINVAL_;
+            }
+            off[3 * p + c] = (int)oc[c];
+        }
+    }
+    for (int w = 0; w < nweights; w++) {
+        const int* o = &off[3 * start[w]];
+        if (o[0] || o[1] || o[2]) {
+            gfd_woperator_free(op);
+            set_error("gfd_woperator_new: stencil %d does not start at the "
+                      "centre", w);
+            return EINVAL_;
+        }
+    }
+    if (hipMalloc(&op->d_start, sizeof(int) * (nweights + 1)) != hipSuccess ||
+        hipMalloc(&op->d_coef, sizeof(double) * total) != hipSuccess ||
+        hipMalloc(&op->d_off, sizeof(int) * 3 * total) != hipSuccess ||
+        hipMalloc((void**)&op->d_wts, sizeof(double*) * nweights) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        gfd_woperator_free(op);
+        set_error("gfd_woperator_new: out of device memory");
+        return ENOMEM_;
+    }
+    if (hipMemcpy(op->d_start, start.data(), sizeof(int) * (nweights + 1),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(op->d_coef, coefs, sizeof(double) * total,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(op->d_off, off.data(), sizeof(int) * 3 * total,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)op->d_wts, weights, sizeof(double*) * nweights,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        gfd_woperator_free(op);
+        set_error("gfd_woperator_new: copy failed");
+        return EHIP;
+    }
+    *out = op;
+    return OK;
+}
+
+extern "C" int gfd_woperator_apply(gfd_woperator* op, const void* in,
+                                   void* out, long nin, const double* phases,
+                                   void* stream)
+{
+    if (!op || nin < 0 || (nin > 0 && (!in || !out)) || in == out) {
+        set_error("gfd_woperator_apply: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) return OK;
+    hipStream_t s = (hipStream_t)stream;
+    const WArgs a = op->args(op->real ? nullptr : phases);
+    const unsigned g = wgrid(op->ng() * nin);
+    if (op->real)
+        hipLaunchKernelGGL(k_wapply<double>, dim3(g), dim3(256), 0, s, a,
+                           (const double*)in, (double*)out, nin);
+    else
+        hipLaunchKernelGGL(k_wapply<double2>, dim3(g), dim3(256), 0, s, a,
+                           (const double2*)in, (double2*)out, nin);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+extern "C" int gfd_woperator_relax(gfd_woperator* op, int method, double* f,
+                                   const double* src, int nrelax, double w,
+                                   void* stream)
+{
+    if (!op || !f || !src || nrelax < 0 || f == src) {
+        set_error("gfd_woperator_relax: bad arguments");
+        return EINVAL_;
+    }
+    if (!op->real) {
+        set_error("gfd_woperator_relax: real data only");
+        return EINVAL_;
+    }
+    if (method != 2) {
+        set_error("gfd_woperator_relax: only weighted Jacobi (method 2) runs "
+                  "on the GPU");
+        return ENOTSUP_;
+    }
+    if (nrelax == 0) return OK;
+    hipStream_t s = (hipStream_t)stream;
+    const long ng = op->ng();
+    GFD_TRY(op->scratch.ensure(sizeof(double) * ng, s));
+    double* tmp = (double*)op->scratch.p;
+    const WArgs a = op->args(nullptr);
+    const unsigned g = wgrid(ng);
+    // sweeps alternate f -> tmp -> f; an odd count ends with a copy back
+    double* cur = f;
+    double* nxt = tmp;
+    for (int k = 0; k < nrelax; k++) {
+        hipLaunchKernelGGL(k_wrelax, dim3(g), dim3(256), 0, s, a,
+                           (const double*)cur, nxt, src, w);
+        GFD_LAUNCH_CHECK();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (cur != f)
+        GFD_CHECK_HIP(hipMemcpyAsync(f, cur, sizeof(double) * ng,
+                                     hipMemcpyDeviceToDevice, s));
+    return OK;
+}
+
+extern "C" int gfd_woperator_get_diagonal_element(gfd_woperator* op,
+                                                  double* out, void* stream)
+{
+    if (!op || !out) {
+        set_error("gfd_woperator_get_diagonal_element: bad arguments");
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned g = std::min(wgrid(op->ng()), 1024u);
+    GFD_TRY(op->partial.ensure(sizeof(double) * g, s));
+    hipLaunchKernelGGL(k_wdiag, dim3(g), dim3(256), 0, s, op->args(nullptr),
+                       (double*)op->partial.p);
+    GFD_LAUNCH_CHECK();
+    std::vector<double> h(g);
+    GFD_CHECK_HIP(hipMemcpyAsync(h.data(), op->partial.p, sizeof(double) * g,
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    double d = 0.0;
+    for (double v : h) d = std::max(d, v);
+    *out = d;
+    return OK;
+}

@@ -324,6 +324,38 @@ int gfd_add_scalar(double* a, long n, double v, void* stream);
  * compared against.  16-byte aligned pointers, nbytes a multiple of 16. */
 int gfd_stream_copy(const void* src, void* dst, long nbytes, void* stream);
 
+/* ------------------------------------------------------ weighted operators
+ * Replaces _gpaw.WOperator(nweights, weights, coefs, offsets, size, range,
+ * neighbors, real, comm, cfd) -- c/woperators.cpp:422-477, bound from
+ * gpaw/wfd_operators.py:63-79 (set_weights).  Stencil w has ncoefs[w]
+ * points, its first at the centre; coefs and offsets (padded strides, as
+ * gfd_operator_new) are concatenated over w; weights is a host array of
+ * nweights device pointers to n0 n1 n2 doubles each, borrowed (not copied)
+ * for the life of the handle, as the reference borrows them. */
+typedef struct gfd_woperator gfd_woperator;
+int gfd_woperator_new(int nweights, const int* ncoefs, const double* coefs,
+                      const long* offsets, const long size[3], int range,
+                      const long neighbors[6], int real,
+                      const double* const* weights, gfd_woperator** out);
+void gfd_woperator_free(gfd_woperator* op);
+
+/* WOperator.apply(in, out[, phases]) -- c/woperators.cpp:212-283 with
+ * c/bmgs/wfd.cpp: out = sum_w weights_w * (stencil_w applied to in). */
+int gfd_woperator_apply(gfd_woperator* op, const void* in, void* out,
+                        long nin, const double* phases, void* stream);
+
+/* WOperator.relax(method, f, src, nrelax[, w]) -- c/woperators.cpp:55-94
+ * with c/bmgs/wrelax.cpp.  Weighted Jacobi (method 2); Gauss-Seidel
+ * (method 1) returns GFD_ENOTSUP, as gfd_operator_relax.  Real only. */
+int gfd_woperator_relax(gfd_woperator* op, int method, double* f,
+                        const double* src, int nrelax, double w,
+                        void* stream);
+
+/* WOperator.get_diagonal_element() -- c/woperators.cpp:341-378: max over
+ * the grid of |sum_w weight_w c_{w,0}| (synchronous, *out on the host). */
+int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
+                                       void* stream);
+
 /* Tuning knobs (no reference counterpart): key "force_path" selects the
  * stencil kernel family (0 automatic, 1 general, 2 tiled x-march, 3
  * full-row x-march; a family that cannot take a grid falls back to the

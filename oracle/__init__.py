@@ -13,6 +13,8 @@ Python restatements of the reference glue that chains them:
 - ``PoissonVCycle``   gpaw/poisson.py:474-537 (solve_neutral / iterate2)
 - ``rmm_block_step``  gpaw/new/pwfd/rmmdiis.py:123-257 (diis_steps=1, grid
                       part: no projector terms)
+- ``wfd_apply``, ``wfd_relax``  c/bmgs/wfd.cpp, c/bmgs/wrelax.cpp (weighted
+                      operators of gpaw/wfd_operators.py), in the C restatement
 
 Pinning (what checks each function; see tests/test_oracle_*.py):
 
@@ -25,6 +27,9 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
 - ``PoissonVCycle``: against the same V-cycle driving the reference C
   operators (fixtures: niter and phi), and against the spectral closed form
   of the converged periodic solution.
+- ``wfd_apply``, ``wfd_relax``: bit-exact against the reference WOperator
+  built from the reference sources (``oracle/_ref``), and with one unit
+  weight against ``fd_apply`` / ``fd_relax``.
 - ``rmm_block_step``: by what the step's algebra fixes -- lambda minimises
   |R + lambda dR| (Re<dR|R + lambda dR> = 0), R + lambda dR is the residual
   of psi + lambda PR (linearity of H), an exact eigenpair is a fixed point,
@@ -67,7 +72,16 @@ def lib():
         L.oracle_transform.argtypes = [
             ctypes.c_int, ctypes.c_int, _lp, _lp, _lp, _lp, _lp, _lp,
             ctypes.c_int, _dp, _dp, _dp, ctypes.c_long, ctypes.c_int]
-        for f in (L.oracle_fd_apply, L.oracle_fd_relax, L.oracle_transform):
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_wfd_apply.argtypes = [
+            ctypes.c_int, _ip, _dp, _lp, _dp, _lp, ctypes.c_int, _lp,
+            ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_long]
+        L.oracle_wfd_relax.argtypes = [
+            ctypes.c_int, ctypes.c_int, _ip, _dp, _lp, _dp, _lp,
+            ctypes.c_int, _lp, ctypes.c_int, _dp, _dp, ctypes.c_int,
+            ctypes.c_double]
+        for f in (L.oracle_fd_apply, L.oracle_fd_relax, L.oracle_transform,
+                  L.oracle_wfd_apply, L.oracle_wfd_relax):
             f.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -148,6 +162,60 @@ def fd_relax(method, coef_p, offset_pc, n_c, neighbor_cd, f, src, nrelax,
                                 _d(f), _d(src), int(nrelax), float(w))
     if err:
         raise ValueError(f'oracle_fd_relax: error {err}')
+    return f
+
+
+def _wstencils(coef_ps, offset_ps):
+    nc = np.ascontiguousarray([len(c) for c in coef_ps], dtype=np.int32)
+    coef = np.ascontiguousarray(np.concatenate(
+        [np.asarray(c, float) for c in coef_ps]))
+    off = _longs(np.concatenate([np.asarray(o).reshape(-1, 3)
+                                 for o in offset_ps]), (-1, 3))
+    return nc, coef, off
+
+
+def wfd_apply(coef_ps, offset_ps, weights, n_c, neighbor_cd, a,
+              phase_cd=None):
+    """Reference WeightedFDOperator.apply (c/woperators.cpp, c/bmgs/wfd.cpp):
+    out = sum_w weights[w] * (stencil w applied to a); every stencil's
+    first point is the centre (gpaw/wfd_operators.py:24-44)."""
+    a = np.ascontiguousarray(a)
+    real = a.dtype == np.float64
+    n_c = _longs(n_c)
+    nc, coef, off = _wstencils(coef_ps, offset_ps)
+    wts = np.ascontiguousarray(np.asarray(weights, float))
+    assert wts.shape == (len(coef_ps),) + tuple(n_c)
+    rng = int(np.abs(off).max())
+    nb = _longs(neighbor_cd)
+    out = np.empty_like(a)
+    nin = int(np.prod(a.shape[:-3], dtype=np.int64))
+    ph, php = _phases(phase_cd, real)
+    err = lib().oracle_wfd_apply(
+        len(nc), nc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(coef),
+        _l(off), _d(wts), _l(n_c), rng, _l(nb), int(real),
+        int(is_cfd(off)), php, _d(a.view(float)), _d(out.view(float)), nin)
+    if err:
+        raise ValueError(f'oracle_wfd_apply: error {err}')
+    return out
+
+
+def wfd_relax(method, coef_ps, offset_ps, weights, n_c, neighbor_cd, f, src,
+              nrelax, w=1.0):
+    """Reference WeightedFDOperator.relax (c/woperators.cpp:55-94,
+    c/bmgs/wrelax.cpp); f updated in place."""
+    assert f.dtype == np.float64 and f.flags.c_contiguous
+    src = np.ascontiguousarray(src, float)
+    n_c = _longs(n_c)
+    nc, coef, off = _wstencils(coef_ps, offset_ps)
+    wts = np.ascontiguousarray(np.asarray(weights, float))
+    rng = int(np.abs(off).max())
+    nb = _longs(neighbor_cd)
+    err = lib().oracle_wfd_relax(
+        int(method), len(nc), nc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+        _d(coef), _l(off), _d(wts), _l(n_c), rng, _l(nb), int(is_cfd(off)),
+        _d(f), _d(src), int(nrelax), float(w))
+    if err:
+        raise ValueError(f'oracle_wfd_relax: error {err}')
     return f
 
 

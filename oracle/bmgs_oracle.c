@@ -287,6 +287,150 @@ int oracle_fd_relax(int method, int ncoefs, const double* coefs,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Weighted operators (c/bmgs/wfd.cpp, c/bmgs/wrelax.cpp)                   */
+/* ------------------------------------------------------------------------ */
+
+static int wsetup(int nweights, const int* ncoefs, const long* off3,
+                  const long n[3], int range, const long nb[6], int cfd,
+                  int nd, halo_t* h, long M[3], long** off)
+{
+    if (nweights <= 0 || range < 0)
+        return 1;
+    long tot = 0;
+    for (int w = 0; w < nweights; w++) {
+        if (ncoefs[w] <= 0 || check_range(ncoefs[w], off3 + 3 * tot, range))
+            return 1;
+        /* the first point of every stencil is the centre */
+        if (off3[3 * tot] != 0 || off3[3 * tot + 1] != 0 ||
+            off3[3 * tot + 2] != 0)
+            return 3;
+        tot += ncoefs[w];
+    }
+    h->cfd = cfd;
+    h->nd = nd;
+    for (int i = 0; i < 3; i++) {
+        h->n[i] = n[i];
+        h->pad[i][0] = h->pad[i][1] = range;
+        h->npad[i][0] = h->npad[i][1] = range;
+        h->nb[i][0] = nb[2 * i];
+        h->nb[i][1] = nb[2 * i + 1];
+    }
+    for (int i = 0; i < 3; i++)
+        M[i] = padded_size(h, i);
+    *off = flat_offsets((int)tot, off3, M);
+    return 0;
+}
+
+int oracle_wfd_apply(int nweights, const int* ncoefs, const double* coefs,
+                     const long* off3, const double* weights, const long n[3],
+                     int range, const long nb[6], int real, int cfd,
+                     const double* phases, const double* in, double* out,
+                     long nin)
+{
+    halo_t h;
+    long M[3];
+    long* off;
+    if (!real && phases == NULL)
+        return 2;
+    int st = wsetup(nweights, ncoefs, off3, n, range, nb, cfd, real ? 1 : 2,
+                    &h, M, &off);
+    if (st)
+        return st;
+    const long np = n[0] * n[1] * n[2];
+    const long ng = np * h.nd;
+    double* buf =
+        (double*)malloc(sizeof(double) * (size_t)(M[0] * M[1] * M[2] * h.nd));
+    for (long m = 0; m < nin; m++) {
+        halo_fill(&h, in + m * ng, buf, real ? NULL : phases);
+        double* o = out + m * ng;
+        for (long i0 = 0; i0 < n[0]; i0++)
+            for (long i1 = 0; i1 < n[1]; i1++)
+                for (long i2 = 0; i2 < n[2]; i2++) {
+                    const long i =
+                        ((i0 + range) * M[1] + (i1 + range)) * M[2] + i2 + range;
+                    const long j = (i0 * n[1] + i1) * n[2] + i2;
+                    double xr = 0.0, xi = 0.0;
+                    long c0 = 0;
+                    for (int w = 0; w < nweights; w++) {
+                        double tr = 0.0, ti = 0.0;
+                        for (int c = 0; c < ncoefs[w]; c++) {
+                            const long k = i + off[c0 + c];
+                            if (real) {
+                                tr += buf[k] * coefs[c0 + c];
+                            } else {
+                                tr += buf[2 * k] * coefs[c0 + c];
+                                ti += buf[2 * k + 1] * coefs[c0 + c];
+                            }
+                        }
+                        const double wt = weights[w * np + j];
+                        tr *= wt;
+                        ti *= wt;
+                        xr += tr;
+                        xi += ti;
+                        c0 += ncoefs[w];
+                    }
+                    if (real) {
+                        o[j] = xr;
+                    } else {
+                        o[2 * j] = xr;
+                        o[2 * j + 1] = xi;
+                    }
+                }
+    }
+    free(buf);
+    free(off);
+    return 0;
+}
+
+int oracle_wfd_relax(int method, int nweights, const int* ncoefs,
+                     const double* coefs, const long* off3,
+                     const double* weights, const long n[3], int range,
+                     const long nb[6], int cfd, double* f, const double* src,
+                     int nrelax, double w)
+{
+    halo_t h;
+    long M[3];
+    long* off;
+    int st = wsetup(nweights, ncoefs, off3, n, range, nb, cfd, 1, &h, M, &off);
+    if (st)
+        return st;
+    const long np = n[0] * n[1] * n[2];
+    double* buf = (double*)malloc(sizeof(double) * (size_t)(M[0] * M[1] * M[2]));
+    for (int sweep = 0; sweep < nrelax; sweep++) {
+        halo_fill(&h, f, buf, NULL);
+        for (long i0 = 0; i0 < n[0]; i0++)
+            for (long i1 = 0; i1 < n[1]; i1++)
+                for (long i2 = 0; i2 < n[2]; i2++) {
+                    const long i = (i0 * n[1] + i1) * n[2] + i2;
+                    const long j =
+                        ((i0 + range) * M[1] + (i1 + range)) * M[2] + i2 + range;
+                    double x = 0.0, coef = 0.0;
+                    long c0 = 0;
+                    for (int q = 0; q < nweights; q++) {
+                        const double wt = weights[q * np + i];
+                        double t = 0.0;
+                        for (int c = 1; c < ncoefs[q]; c++)
+                            t += buf[j + off[c0 + c]] * coefs[c0 + c];
+                        t *= wt;
+                        x += t;
+                        coef += wt * coefs[c0];
+                        c0 += ncoefs[q];
+                    }
+                    if (method == 1) {
+                        x = (src[i] - x) / coef;
+                        f[i] = x;
+                        buf[j] = x;
+                    } else {
+                        f[i] = (1.0 - w) * f[i] + w * (src[i] - x) / coef;
+                    }
+                }
+    }
+    free(buf);
+    free(off);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Restriction and interpolation                                            */
 /* ------------------------------------------------------------------------ */
 

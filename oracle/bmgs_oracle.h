@@ -32,6 +32,24 @@ int oracle_fd_relax(int method, int ncoefs, const double* coefs,
                     const long nb[6], int cfd, double* f, const double* src,
                     int nrelax, double w);
 
+/* WeightedFDOperator.apply (c/bmgs/wfd.cpp): out = sum_w weights[w](x) *
+ * sum_p coefs_w[p] in(x + off_w[p]); stencil w has ncoefs[w] points (the
+ * first at the centre), coefs/off3 concatenated over w, weights nweights
+ * grids of n0 n1 n2 doubles. */
+int oracle_wfd_apply(int nweights, const int* ncoefs, const double* coefs,
+                     const long* off3, const double* weights, const long n[3],
+                     int range, const long nb[6], int real, int cfd,
+                     const double* phases, const double* in, double* out,
+                     long nin);
+
+/* WeightedFDOperator.relax (c/bmgs/wrelax.cpp): Jacobi (method 2) or
+ * Gauss-Seidel (method 1) sweeps, real data, f updated in place. */
+int oracle_wfd_relax(int method, int nweights, const int* ncoefs,
+                     const double* coefs, const long* off3,
+                     const double* weights, const long n[3], int range,
+                     const long nb[6], int cfd, double* f, const double* src,
+                     int nrelax, double w);
+
 /* Transformer.apply: restriction (interpolate == 0) or interpolation with a
  * k-tap (k = 2 nn) stencil; pad/npad/skip are the (3, 2) arrays the Python
  * front end computes. */

@@ -1,0 +1,76 @@
+"""Weighted FD operators (gpaw/wfd_operators.py; c/bmgs/wfd.cpp,
+wrelax.cpp) against the oracle, which equals the reference WOperator bit
+for bit (test_oracle_ref.py).  The operator of the dielectric Poisson
+solver: eps * Laplace + sum_v d_v eps * Gradient_v, nn=3."""
+import numpy as np
+import pytest
+
+import oracle
+from gpaw_amd import _lib
+from gpaw_amd.fd_operators import Gradient, Laplace
+from gpaw_amd.grid_descriptor import GridDescriptor
+from gpaw_amd.wfd_operators import WeightedFDOperator
+from gpu_util import dev, host, say
+from helpers import bloch_phases, rand, relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def dielectric_op(gd, dtype=float, nn=3):
+    ops = [Laplace(gd, 1.0, nn, dtype)]
+    ops += [Gradient(gd, v, 1.0, nn, dtype) for v in range(3)]
+    return WeightedFDOperator(ops)
+
+
+def weights_of(rng, n):
+    eps = 1.0 + rng.uniform(0, 78, n)
+    return [eps] + [rng.standard_normal(n) for _ in range(3)]
+
+
+@pytest.mark.parametrize('pbc', [True, False, (True, False, True)])
+@pytest.mark.parametrize('dtype', [float, complex])
+def test_apply(pbc, dtype):
+    rng = np.random.default_rng(11)
+    gd = GridDescriptor((16, 12, 20), (4.0, 3.0, 5.0), pbc)
+    n = tuple(int(x) for x in gd.n_c)
+    A = dielectric_op(gd, dtype)
+    w = weights_of(rng, n)
+    A.set_weights([dev(x) for x in w])
+    a = rand(rng, (3,) + n, dtype)
+    ph = bloch_phases(rng, gd.pbc_c) if dtype == complex else None
+    ref = oracle.wfd_apply(A.coef_ps, A.offset_pcs, np.array(w), n,
+                           gd.neighbor_cd, a, ph)
+    say('wfd apply', pbc, dtype.__name__)
+    d = dev(a)
+    out = d.new_empty(d.shape)
+    A.apply(d, out, ph)
+    assert relerr(host(out), ref) < TOL
+    # numpy arrays staged through the device give the same result
+    outh = np.empty_like(a)
+    A.apply(a, outh, ph)
+    assert relerr(outh, ref) < TOL
+
+
+@pytest.mark.parametrize('pbc', [True, False])
+def test_relax_and_diagonal(pbc):
+    rng = np.random.default_rng(12)
+    gd = GridDescriptor((16, 16, 12), (4.0, 4.0, 3.0), pbc)
+    n = tuple(int(x) for x in gd.n_c)
+    A = dielectric_op(gd)
+    w = weights_of(rng, n)
+    A.set_weights([dev(x) for x in w])
+    f = rng.standard_normal(n)
+    src = rng.standard_normal(n)
+    for nrelax in (1, 4):
+        fr = f.copy()
+        oracle.wfd_relax(2, A.coef_ps, A.offset_pcs, np.array(w), n,
+                         gd.neighbor_cd, fr, src, nrelax, 0.6)
+        say('wfd relax', pbc, nrelax)
+        fd = dev(f)
+        A.relax(2, fd, dev(src), nrelax, 0.6)
+        assert relerr(host(fd), fr) < TOL
+    diag = abs(sum(wt * c[0] for wt, c in zip(w, A.coef_ps))).max()
+    assert A.get_diagonal_element() == pytest.approx(diag, rel=1e-14)
+    with pytest.raises(NotImplementedError):
+        A.relax(1, dev(f), dev(src), 1)

@@ -116,3 +116,71 @@ def test_random_transfers(G):
                 assert (b1 == b2).all(), (trial, interp, real, nn)
                 count += 1
     assert count > 100
+
+
+def test_weighted_operator_apply_and_relax(G):
+    """The oracle's weighted apply / relax (c/bmgs/wfd.cpp, wrelax.cpp)
+    equal the reference WOperator bit for bit, and with one unit weight
+    equal the unweighted operator."""
+    rng = np.random.default_rng(7)
+    for trial in range(12):
+        n = rng.integers(4, 10, 3)
+        r = int(rng.integers(1, 4))
+        nw = int(rng.integers(1, 5))
+        coef_ps, off_ps = [], []
+        for w in range(nw):
+            P = int(rng.integers(2, 8))
+            off3 = np.zeros((P, 3), int)
+            for p in range(1, P):
+                ax = rng.integers(0, 3)
+                off3[p, ax] = rng.choice([k for k in range(-r, r + 1) if k])
+            coef_ps.append(rng.standard_normal(P))
+            off_ps.append(off3)
+        # the diagonal must dominate for the relaxation
+        for c in coef_ps:
+            c[0] = -abs(c).sum() * 2
+        per = [0 if rng.random() < .6 else -1 for _ in range(3)]
+        nb = np.array([[p, p] for p in per])
+        weights = rng.uniform(0.5, 1.5, (nw,) + tuple(n))
+        M = np.array(n) + 2 * r
+        st = np.array([M[1] * M[2], M[2], 1])
+        wl = [np.ascontiguousarray(w) for w in weights]
+        for real in (1, 0):
+            cfd = int(oracle.is_cfd(np.concatenate(off_ps)))
+            op = G.WOperator(nw, wl, [np.asarray(c, float) for c in coef_ps],
+                             [np.ascontiguousarray(o @ st) for o in off_ps],
+                             np.array(n), r, nb, real, None, cfd)
+            shp = (2,) + tuple(n)
+            a = rng.standard_normal(shp)
+            ph = None
+            if not real:
+                a = a + 1j * rng.standard_normal(shp)
+                ph = np.exp(2j * np.pi * rng.random((3, 1)) * np.array([1, -1]))
+            b1 = np.empty_like(a)
+            if real:
+                op.apply(a, b1)
+            else:
+                op.apply(a, b1, ph)
+            b2 = oracle.wfd_apply(coef_ps, off_ps, weights, n, nb, a, ph)
+            assert (b1 == b2).all(), trial
+        for method in (1, 2):
+            f1 = rng.standard_normal(tuple(n))
+            f2 = f1.copy()
+            src = rng.standard_normal(tuple(n))
+            op = G.WOperator(nw, wl, [np.asarray(c, float) for c in coef_ps],
+                             [np.ascontiguousarray(o @ st) for o in off_ps],
+                             np.array(n), r, nb, 1, None,
+                             int(oracle.is_cfd(np.concatenate(off_ps))))
+            op.relax(method, f1, src, 3, 0.7)
+            oracle.wfd_relax(method, coef_ps, off_ps, weights, n, nb, f2,
+                             src, 3, 0.7)
+            assert (f1 == f2).all(), (trial, method)
+    # one unit weight: the plain operator
+    n = (6, 7, 8)
+    off3 = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 2, 0], [0, 0, -1]])
+    coef = rng.standard_normal(5)
+    nb = np.array([[0, 0], [-1, -1], [0, 0]])
+    a = rng.standard_normal((2,) + n)
+    one = np.ones((1,) + n)
+    assert (oracle.wfd_apply([coef], [off3], one, n, nb, a) ==
+            oracle.fd_apply(coef, off3, n, nb, a)).all()
