@@ -20,6 +20,8 @@
 
 namespace gfd {
 
+constexpr int WMAXW = 64;  // weights per operator
+
 struct WArgs {
     int nw;
     int n[3];
@@ -29,6 +31,10 @@ struct WArgs {
     const int* off;            // [points][3]
     const double* const* wts;  // [nw] device pointers, ng doubles each
     BC bc;
+    // flat offset of the (x, y) part of every point (points away from
+    // the x and y edges) and the largest offset component
+    const long* foff;
+    int R;
 };
 
 template <typename T>
@@ -64,14 +70,38 @@ k_wapply(WArgs a, const T* __restrict__ in, T* __restrict__ out, long nin)
         const int i1 = (int)(q % a.n[1]), i0 = (int)(q / a.n[1]);
         const T* g = in + b * a.ng;
         T x = zero_of<T>();
-        for (int w = 0; w < a.nw; w++) {
-            T t = zero_of<T>();
-            for (int c = a.start[w]; c < a.start[w + 1]; c++) {
-                bool ok;
-                const T v = wneighbour(a, g, i0, i1, i2, c, ok);
-                if (ok) t = fmac(t, (S)a.coef[c], v);
+        const int R = a.R;
+        if (i0 >= R && i0 < a.n[0] - R && i1 >= R && i1 < a.n[1] - R) {
+            // away from the x and y edges (uniform across most waves): flat
+            // (x, y) offsets, z wrapped by selects
+            const long row = idx - i2;
+            for (int w = 0; w < a.nw; w++) {
+                T t = zero_of<T>();
+                for (int c = a.start[w]; c < a.start[w + 1]; c++) {
+                    int j2 = i2 + a.off[3 * c + 2];
+                    double2 f = make_double2(1.0, 0.0);
+                    bool ph = false;
+                    if (j2 < 0 || j2 >= a.n[2]) {
+                        if (!a.bc.periodic[2]) continue;
+                        ph = true;
+                        f = j2 < 0 ? a.bc.phase[2][1] : a.bc.phase[2][0];
+                        j2 = j2 < 0 ? j2 + a.n[2] : j2 - a.n[2];
+                    }
+                    const T v = scale_by<T>(g[row + a.foff[c] + j2], f, ph);
+                    t = fmac(t, (S)a.coef[c], v);
+                }
+                x = fmac(x, (S)a.wts[w][idx], t);
             }
-            x = fmac(x, (S)a.wts[w][idx], t);
+        } else {
+            for (int w = 0; w < a.nw; w++) {
+                T t = zero_of<T>();
+                for (int c = a.start[w]; c < a.start[w + 1]; c++) {
+                    bool ok;
+                    const T v = wneighbour(a, g, i0, i1, i2, c, ok);
+                    if (ok) t = fmac(t, (S)a.coef[c], v);
+                }
+                x = fmac(x, (S)a.wts[w][idx], t);
+            }
         }
         out[p] = x;
     }
@@ -90,13 +120,21 @@ k_wrelax(WArgs a, const double* __restrict__ cur, double* __restrict__ nxt,
         const long q = idx / a.n[2];
         const int i1 = (int)(q % a.n[1]), i0 = (int)(q / a.n[1]);
         double x = 0.0, coef = 0.0;
+        const int R = a.R;
+        const bool inner = i0 >= R && i0 < a.n[0] - R && i1 >= R &&
+                           i1 < a.n[1] - R && i2 >= R && i2 < a.n[2] - R;
         for (int v = 0; v < a.nw; v++) {
             const double wt = a.wts[v][idx];
             double t = 0.0;
             for (int c = a.start[v] + 1; c < a.start[v + 1]; c++) {
-                bool ok;
-                const double nb = wneighbour(a, cur, i0, i1, i2, c, ok);
-                if (ok) t = fma(a.coef[c], nb, t);
+                if (inner) {
+                    t = fma(a.coef[c],
+                            cur[idx + a.foff[c] + a.off[3 * c + 2]], t);
+                } else {
+                    bool ok;
+                    const double nb = wneighbour(a, cur, i0, i1, i2, c, ok);
+                    if (ok) t = fma(a.coef[c], nb, t);
+                }
             }
             x = fma(wt, t, x);
             coef = fma(wt, a.coef[a.start[v]], coef);
@@ -148,6 +186,8 @@ struct gfd_woperator {
     double* d_coef = nullptr;
     int* d_off = nullptr;
     const double** d_wts = nullptr;
+    int R = 0;
+    long* d_foff = nullptr;
     DevBuf scratch;
     DevBuf partial;
     long ng() const { return (long)n[0] * n[1] * n[2]; }
@@ -161,6 +201,8 @@ struct gfd_woperator {
         a.coef = d_coef;
         a.off = d_off;
         a.wts = d_wts;
+        a.R = R;
+        a.foff = d_foff;
         for (int c = 0; c < 3; c++) {
             a.bc.periodic[c] = periodic[c];
             for (int d = 0; d < 2; d++)
@@ -180,6 +222,7 @@ extern "C" void gfd_woperator_free(gfd_woperator* op)
     if (op->d_coef) (void)hipFree(op->d_coef);
     if (op->d_off) (void)hipFree(op->d_off);
     if (op->d_wts) (void)hipFree((void*)op->d_wts);
+    if (op->d_foff) (void)hipFree(op->d_foff);
     op->scratch.release();
     op->partial.release();
     delete op;
@@ -192,8 +235,8 @@ extern "C" int gfd_woperator_new(int nweights, const int* ncoefs,
                                  const double* const* weights,
                                  gfd_woperator** out)
 {
-    if (!out || nweights <= 0 || !ncoefs || !coefs || !offsets || !size ||
-        !neighbors || !weights || range < 0) {
+    if (!out || nweights <= 0 || nweights > WMAXW || !ncoefs || !coefs ||
+        !offsets || !size || !neighbors || !weights || range < 0) {
         set_error("gfd_woperator_new: bad arguments");
         return EINVAL_;
     }
@@ -263,6 +306,23 @@ extern "C" int gfd_woperator_new(int nweights, const int* ncoefs,
                       "centre", w);
             return EINVAL_;
         }
+    }
+    // flat offsets for the interior path
+    std::vector<long> fo(total);
+    int R = 0;
+    for (long p = 0; p < total; p++) {
+        const int* o = &off[3 * p];
+        fo[p] = ((long)o[0] * size[1] + o[1]) * size[2];
+        for (int c = 0; c < 3; c++) R = std::max(R, std::abs(o[c]));
+    }
+    op->R = R;
+    if (hipMalloc(&op->d_foff, sizeof(long) * total) != hipSuccess ||
+        hipMemcpy(op->d_foff, fo.data(), sizeof(long) * total,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        gfd_woperator_free(op);
+        set_error("gfd_woperator_new: out of device memory");
+        return ENOMEM_;
     }
     if (hipMalloc(&op->d_start, sizeof(int) * (nweights + 1)) != hipSuccess ||
         hipMalloc(&op->d_coef, sizeof(double) * total) != hipSuccess ||
