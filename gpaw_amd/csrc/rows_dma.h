@@ -455,6 +455,9 @@ static int launch_xd(const AxisStencil& st, const XmTile& tl, const int n[3],
     XD_CASE(3, 1, 2) XD_CASE(3, 2, 2) XD_CASE(3, 4, 2)
     XD_CASE(4, 1, 1) XD_CASE(4, 2, 1) XD_CASE(4, 4, 1)
     XD_CASE(4, 1, 2) XD_CASE(4, 2, 2) XD_CASE(4, 4, 2)
+    // 3 rows per thread: 24-row tiles with 8 full waves of 48-thread rows
+    XD_CASE(1, 3, 1) XD_CASE(1, 3, 2) XD_CASE(2, 3, 1) XD_CASE(2, 3, 2)
+    XD_CASE(3, 3, 1) XD_CASE(3, 3, 2) XD_CASE(4, 3, 1) XD_CASE(4, 3, 2)
 #undef XD_CASE
     set_error("x-dma: unsupported radius %d / rows %d", st.R, tl.py);
     return ENOTSUP_;

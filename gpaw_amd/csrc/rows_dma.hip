@@ -28,7 +28,7 @@ bool row_dma_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
     const int es = dtype == 0 ? 8 : 4;
     const int pz = 16 / es;
     return xd_lds_bytes(t, R, es, n) > 0 && n[1] % (t.nseg * t.py) == 0 &&
-           (t.py == 1 || t.py == 2 || t.py == 4) && R <= n[0] && R <= n[1] &&
+           (t.py >= 1 && t.py <= 4) && R <= n[0] && R <= n[1] &&
            t.xchunk >= 1 && n[2] >= ((R + pz - 1) / pz) * pz;
 }
 

@@ -245,7 +245,7 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
         // the whole grid (measured on 256 x 96^3 f64 nn=3: tall tiles and
         // whole-grid chunks win, 8 x 48 threads x 4 rows, 2 planes ahead)
         static const int dxcs[] = {16, 32, 1 << 30};
-        for (int py = 1; py <= 4; py <<= 1)
+        for (int py = 1; py <= 4; py++)
             for (int sg = 1; sg * TZT <= XR_MAXT; sg++) {
                 if (n[1] % (sg * py) != 0 || sg * TZT < 64) continue;
                 for (int A = 1; A <= 2; A++)

@@ -1,6 +1,6 @@
 """The LDS-DMA row kernel (k_xdma: plain passes on real data) against the
 oracle, on tiles forced through the tile override: every radius, rows per
-thread 1/2/4, 1-2 planes in flight, x chunks that do and do not divide the
+thread 1-4, 1-2 planes in flight, x chunks that do and do not divide the
 grid, periodic and zero boundaries.  Tolerance 1e-13 relative max-norm."""
 import numpy as np
 import pytest
@@ -37,7 +37,7 @@ def tile_code(n2, nseg, py, xchunk, A):
 
 
 def tiles_for(n, R, rng, count=3):
-    cands = [(s, p, A) for s in range(1, 9) for p in (1, 2, 4)
+    cands = [(s, p, A) for s in range(1, 9) for p in (1, 2, 3, 4)
              for A in (1, 2) if tile_ok(n, R, s, p, A)]
     pick = rng.choice(len(cands), size=min(count, len(cands)),
                       replace=False)
