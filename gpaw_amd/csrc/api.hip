@@ -1546,6 +1546,7 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         t.xchunk = (int)((value >> 32) & 0xffff);
         t.dma = (int)((value >> 48) & 0xf);
         t.order = (int)((value >> 52) & 0xf);
+        t.wide = (int)((value >> 56) & 0xf);
         if (t.tz <= 0 || t.nseg <= 0 || t.py <= 0 || t.xchunk <= 0) {
             set_error("gfd_set_tuning: bad xr_tile");
             return EINVAL_;

@@ -204,7 +204,10 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
 #pragma unroll
         for (int j = 0; j < PY; j++) {
             if (a.nt & 2)
-                dst[j] = ld16nt(reinterpret_cast<const C*>(p + j * n2));
+                if constexpr (sizeof(C) == 16)
+                    dst[j] = ld16nt(reinterpret_cast<const C*>(p + j * n2));
+                else
+                    dst[j] = *reinterpret_cast<const C*>(p + j * n2);
             else
                 dst[j] = *reinterpret_cast<const C*>(p + j * n2);
             if constexpr (IsCplx<T>::value)
@@ -379,7 +382,10 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
                                                        a.s, a.negate, nrm);
                     }
                     if (a.nt & 1)
-                        st16nt(r, reinterpret_cast<C*>(a.out + idx));
+                        if constexpr (sizeof(C) == 16)
+                            st16nt(r, reinterpret_cast<C*>(a.out + idx));
+                        else
+                            *reinterpret_cast<C*>(a.out + idx) = r;
                     else
                         *reinterpret_cast<C*>(a.out + idx) = r;
                 }
@@ -392,13 +398,16 @@ __global__ void __launch_bounds__(XR_MAXT) k_xrow(XrArgs<T> a)
 
 template <typename T> struct PzOf { static constexpr int value = 16 / sizeof(T); };
 
-template <typename T, int R, int PY, int EPI>
+// WIDE = 2: two 16-byte points per thread along z for 16-byte data
+// (complex128), so a 128-point row takes 64 threads and a block can hold
+// taller tiles
+template <typename T, int R, int PY, int EPI, int WIDE = 1>
 static int launch_xr_t(const AxisStencil& st, const XmTile& tl,
                        const int n[3], const BC& bc, const void* in,
                        void* out, long nin, const EpiArgs& ea,
                        hipStream_t stream, int* nblocks_out)
 {
-    constexpr int PZ = PzOf<T>::value;
+    constexpr int PZ = PzOf<T>::value * WIDE;
     XrArgs<T> a;
     a.in = (const T*)in;
     a.out = (T*)out;
@@ -466,6 +475,16 @@ static int launch_xr_py(const AxisStencil& st, const XmTile& tl,
                         void* out, long nin, const EpiArgs& ea,
                         hipStream_t stream, int* nbo)
 {
+    if constexpr (sizeof(T) == 16) {
+        if (tl.wide == 2) {
+            switch (tl.py) {
+            case 1: return launch_xr_t<T, R, 1, EPI, 2>(st, tl, n, bc, in, out, nin, ea, stream, nbo);
+            case 2: return launch_xr_t<T, R, 2, EPI, 2>(st, tl, n, bc, in, out, nin, ea, stream, nbo);
+            }
+            set_error("x-row: unsupported rows per thread %d (wide)", tl.py);
+            return ENOTSUP_;
+        }
+    }
     switch (tl.py) {
     case 1: return launch_xr_t<T, R, 1, EPI>(st, tl, n, bc, in, out, nin, ea, stream, nbo);
     case 2: return launch_xr_t<T, R, 2, EPI>(st, tl, n, bc, in, out, nin, ea, stream, nbo);

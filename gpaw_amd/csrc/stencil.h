@@ -61,6 +61,7 @@ struct XmTile {
     int dma = 0; // full-row kernel: planes in flight by LDS DMA (k_xdma),
                  // 0 for register prefetch (k_xrow)
     int order = 0;  // k_xdma block order (rows_dma.h)
+    int wide = 0;   // k_xrow on 16-byte data: 2 = two points per thread
 };
 // Launch one stencil pass over `nin` grids of shape n[3].
 // dtype: 0 = f64, 1 = c128, 2 = f32, 3 = c64 (row kernel and general

@@ -16,7 +16,8 @@ static int xr_pz(int dtype) { return 16 / xr_es(dtype); }
 
 static bool xr_tile_ok(const XmTile& t, int R, int dtype, const int n[3])
 {
-    const int PZ = xr_pz(dtype);
+    if (t.wide == 2 && (dtype != 1 || t.py > 2)) return false;
+    const int PZ = xr_pz(dtype) * (t.wide == 2 ? 2 : 1);
     if (n[2] % PZ != 0 || t.tz != n[2]) return false;
     const int TZT = n[2] / PZ;
     const int NT = TZT * t.nseg;
@@ -240,6 +241,26 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
             t.xchunk = n[0];
             if (xr_tile_ok(t, R, dtype, n)) cand.push_back(t);
         }
+    if (dtype == 1 && n[2] % 2 == 0) {
+        // two complex points per thread: half the threads per row, so
+        // taller tiles (fewer halo rows per output row)
+        const int TZW = n[2] / 2;
+        for (int py = 1; py <= 2; py++)
+            for (int sg = (2 * R + XR_HROWS - 1) / XR_HROWS; sg * TZW <= XR_MAXT;
+                 sg++) {
+                if (n[1] % (sg * py) != 0 || sg * TZW < 64) continue;
+                for (int xc : {16, 32, n[0]}) {
+                    if (xc < std::max(2 * R, XR_XCMIN) || xc > n[0]) continue;
+                    XmTile t;
+                    t.tz = n[2];
+                    t.nseg = sg;
+                    t.py = py;
+                    t.xchunk = xc;
+                    t.wide = 2;
+                    if (xr_tile_ok(t, R, dtype, n)) cand.push_back(t);
+                }
+            }
+    }
     if (dma && (dtype == 0 || dtype == 2)) {
         // LDS-DMA ring: 1-2 planes in flight, x chunks of 16-64 planes or
         // the whole grid (measured on 256 x 96^3 f64 nn=3: tall tiles and

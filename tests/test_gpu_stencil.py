@@ -189,3 +189,43 @@ def test_row_kernel_default_tiles_large_batch():
     for band in (0, nb - 1):
         ref = ref_apply(op, host(d[band]))
         assert relerr(host(out[band]), ref) < TOL
+
+
+@pytest.mark.parametrize('pbc', [True, False, (True, False, True)])
+def test_row_kernel_wide_complex_tiles(pbc):
+    """Full-row tiles with two complex points per thread (forced through
+    the tile override), plain apply with Bloch phases and the H apply
+    epilogue, against the oracle."""
+    import torch
+    from gpaw_amd import _lib
+    from gpaw_amd.bands import apply_hamiltonian
+    lib = _lib.load()
+    rng = np.random.default_rng(90)
+    per = (pbc,) * 3 if isinstance(pbc, bool) else pbc
+    # interior sizes (12, 12, 16) whatever the boundaries
+    N = tuple(m + (0 if p else 1) for m, p in zip((12, 12, 16), per))
+    gd = GridDescriptor(N, (3.0, 3.0, 4.0), pbc)
+    n = tuple(int(x) for x in gd.n_c)
+    assert n == (12, 12, 16)
+    op = Laplace(gd, -0.5, 3, complex)
+    a = rand(rng, (4,) + n, complex)
+    ph = bloch_phases(rng, gd.pbc_c)
+    ref = ref_apply(op, a, ph)
+    vt = rng.standard_normal(n)
+    for nseg, py in [(2, 1), (3, 1), (4, 1), (2, 2)]:
+        if n[1] % (nseg * py) or (n[2] // 2) * nseg > 512:
+            continue
+        code = n[2] | (nseg << 12) | (py << 24) | (5 << 32) | (2 << 56)
+        try:
+            _lib.check(lib.gfd_set_tuning(b'xr_tile', code), 'tuning')
+            with kernel_path('row'):
+                say('wide c128 tile', n, pbc, nseg, py)
+                out = gpu_apply(op, a, ph)
+                d = dev(a)
+                h = d.new_empty(d.shape)
+                apply_hamiltonian(op, torch.tensor(vt, device=d.device), d,
+                                  h, ph)
+        finally:
+            lib.gfd_set_tuning(b'xr_tile', 0)
+        assert relerr(out, ref) < TOL, (nseg, py)
+        assert relerr(host(h), ref + a * vt) < TOL, (nseg, py)
