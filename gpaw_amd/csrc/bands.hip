@@ -84,23 +84,22 @@ __global__ void __launch_bounds__(512) k_gram(GramArgs a)
             acci[r][c] = d4{0.0, 0.0, 0.0, 0.0};
         }
 
-    // stage loader: 128 rows x 16 points of A and of B, 4 per thread; the
-    // next stage's values are fetched into registers while the current
-    // stage is multiplied out of LDS
-    const int lrow = t >> 2, lkq = (t & 3) * 4;
-    const bool arow = i0 + lrow < a.M, brow = j0 + lrow < a.N;
-    const double2* pa = a.A + (long)(i0 + lrow) * a.lda;
-    const double2* pb = a.B + (long)(j0 + lrow) * a.ldb;
+    // stage loader: 128 rows x 16 points of A and of B, 4 rows per thread
+    // (16 lanes read 16 consecutive points of a row: 256-byte segments,
+    // conflict-free LDS writes); the next stage's values are fetched into
+    // registers while the current stage is multiplied out of LDS
+    const int lk16 = t & 15, lr = t >> 4;
     double2 va[4], vb[4];
     auto fetch = [&](long ks) {
+        const long k = ks + lk16;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const long k = ks + lkq + q;
+            const int row = lr + 32 * q;
             va[q] = make_double2(0.0, 0.0);
             vb[q] = make_double2(0.0, 0.0);
             if (k < ke) {
-                if (arow) va[q] = pa[k];
-                if (brow) vb[q] = pb[k];
+                if (i0 + row < a.M) va[q] = a.A[(long)(i0 + row) * a.lda + k];
+                if (j0 + row < a.N) vb[q] = a.B[(long)(j0 + row) * a.ldb + k];
             }
         }
     };
@@ -108,10 +107,10 @@ __global__ void __launch_bounds__(512) k_gram(GramArgs a)
     for (long ks = kb; ks < ke; ks += GK) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            sAr[lkq + q][lrow] = va[q].x;
-            sAi[lkq + q][lrow] = va[q].y;
-            sBr[lkq + q][lrow] = vb[q].x;
-            sBi[lkq + q][lrow] = vb[q].y;
+            sAr[lk16][lr + 32 * q] = va[q].x;
+            sAi[lk16][lr + 32 * q] = va[q].y;
+            sBr[lk16][lr + 32 * q] = vb[q].x;
+            sBi[lk16][lr + 32 * q] = vb[q].y;
         }
         __syncthreads();
         if (ks + GK < ke) fetch(ks + GK);
@@ -199,21 +198,23 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
             accr[r][c] = d4{0.0, 0.0, 0.0, 0.0};
             acci[r][c] = d4{0.0, 0.0, 0.0, 0.0};
         }
-    // C tile: 128 rows x 16 columns, 4 per thread; psi tile: 16 rows x 128
-    // points, 4 consecutive points per thread
-    const int crow = t >> 2, cq = (t & 3) * 4;
-    const int prow = t >> 5, pq = (t & 31) * 4;
-    const int gi = i0 + crow;
+    // C tile: 128 rows x 16 columns, one column of 4 rows per thread; psi
+    // tile: 16 rows x 128 points, one point of 4 rows per thread
+    // (consecutive lanes on consecutive columns / points: conflict-free LDS
+    // writes)
+    const int ck = t & 15, cr = t >> 4;
+    const int pp = t & 127, pr = t >> 7;
     double2 vc[4], vp[4];
     auto fetch = [&](int ks) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int j = ks + cq + q;
+            const int gi = i0 + cr + 32 * q;
+            const int j = ks + ck;
             vc[q] = (gi < a.n && j < kend && j <= gi)
                         ? a.C[(long)gi * a.n + j]
                         : make_double2(0.0, 0.0);
-            const int jp = ks + prow;
-            const long x = x0 + pq + q;
+            const int jp = ks + pr + 4 * q;
+            const long x = x0 + pp;
             vp[q] = (jp < kend && x < a.X) ? a.psi[(long)jp * a.ldpsi + x]
                                            : make_double2(0.0, 0.0);
         }
@@ -222,10 +223,10 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
     for (int ks = 0; ks < kend; ks += GK) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            sCr[cq + q][crow] = vc[q].x;
-            sCi[cq + q][crow] = vc[q].y;
-            sPr[prow][pq + q] = vp[q].x;
-            sPi[prow][pq + q] = vp[q].y;
+            sCr[ck][cr + 32 * q] = vc[q].x;
+            sCi[ck][cr + 32 * q] = vc[q].y;
+            sPr[pr + 4 * q][pp] = vp[q].x;
+            sPi[pr + 4 * q][pp] = vp[q].y;
         }
         __syncthreads();
         if (ks + GK < kend) fetch(ks + GK);
