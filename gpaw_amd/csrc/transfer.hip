@@ -301,6 +301,57 @@ __global__ void __launch_bounds__(256) k_interp2(F3Args<T, TI> a)
     f3store<T, EPI, TI>(a, gid, v);
 }
 
+// value of interpolated output point (o0, o1, o2) of grid g
+template <typename T, typename TI>
+__device__ inline T interp2_point(const F3Args<T, TI>& a,
+                                  const TI* __restrict__ g, int o0, int o1,
+                                  int o2)
+{
+    typedef typename ScalarOf<T>::type S;
+    const int q0 = o0 + a.skip0[0], q1 = o1 + a.skip0[1], q2 = o2 + a.skip0[2];
+    const int c0 = (q0 >> 1) + a.off[0], c1 = (q1 >> 1) + a.off[1],
+              c2 = (q2 >> 1) + a.off[2];
+    const int n0 = (q0 & 1) + 1, n1 = (q1 & 1) + 1, n2 = (q2 & 1) + 1;
+    T vx[2];
+#pragma unroll
+    for (int dx = 0; dx < 2; dx++) {
+        if (dx >= n0) break;
+        T vy[2];
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            if (dy >= n1) break;
+            T vz = f3load(a, g, c0 + dx, c1 + dy, c2);
+            if (n2 == 2)
+                vz = (S)0.5 * (vz + f3load(a, g, c0 + dx, c1 + dy, c2 + 1));
+            vy[dy] = vz;
+        }
+        vx[dx] = (n1 == 2) ? (S)0.5 * (vy[0] + vy[1]) : vy[0];
+    }
+    return (n0 == 2) ? (S)0.5 * (vx[0] + vx[1]) : vx[0];
+}
+
+// Two neighbouring outputs along z per thread (even output rows): the
+// index arithmetic is shared and the coarse values both need are loaded
+// once
+template <typename T, int EPI, typename TI>
+__global__ void __launch_bounds__(256) k_interp2p(F3Args<T, TI> a)
+{
+    const long gp = (long)blockIdx.x * 256 + threadIdx.x;
+    if (2 * gp >= a.total) return;
+    const long gid = 2 * gp;
+    const long b = gid / a.out_grid;
+    long r = gid - b * a.out_grid;
+    const int o2 = (int)(r % a.no[2]);
+    r /= a.no[2];
+    const int o1 = (int)(r % a.no[1]);
+    const int o0 = (int)(r / a.no[1]);
+    const TI* __restrict__ g = a.in + b * a.in_grid;
+    const T v0 = interp2_point(a, g, o0, o1, o2);
+    const T v1 = interp2_point(a, g, o0, o1, o2 + 1);
+    f3store<T, EPI, TI>(a, gid, v0);
+    f3store<T, EPI, TI>(a, gid + 1, v1);
+}
+
 template <typename T, int EPI, typename TI>
 static int launch_fused2_t(const XferPlan& p, const BC& bc, const void* in,
                            void* out, long nin, const XEpiArgs& ea,
@@ -331,7 +382,11 @@ static int launch_fused2_t(const XferPlan& p, const BC& bc, const void* in,
         set_error("transfer: too many blocks");
         return ENOTSUP_;
     }
-    if (p.interpolate)
+    if (p.interpolate && p.nout[2] % 2 == 0)
+        hipLaunchKernelGGL((k_interp2p<T, EPI, TI>),
+                           dim3((unsigned)((a.total / 2 + 255) / 256)),
+                           dim3(256), 0, s, a);
+    else if (p.interpolate)
         hipLaunchKernelGGL((k_interp2<T, EPI, TI>), dim3((unsigned)nb),
                            dim3(256), 0, s, a);
     else
