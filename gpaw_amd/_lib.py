@@ -95,6 +95,7 @@ SIGNATURES = {
     'gfd_add_scalar': (_i, [_vp, _l, _d, _vp]),
     'gfd_stream_copy': (_i, [_vp, _vp, _l, _vp]),
     'gfd_set_tuning': (_i, [ctypes.c_char_p, _l]),
+    'gfd_row_tile_candidates': (_i, [_i, _i, _lp, _i, _lp, _i]),
 }
 
 _lib = None

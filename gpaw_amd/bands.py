@@ -29,16 +29,34 @@ def _npts(psi):
     return int(np.prod(psi.shape[1:]))
 
 
+def _bands(x, name, dtype=None):
+    """Check a (nbands, ...) device tensor before its pointer reaches the
+    library: the kernels read the dtype's element size over the whole
+    shape, so a wrong dtype or a strided view would read out of bounds."""
+    if not (torch is not None and isinstance(x, torch.Tensor)) or \
+            x.device.type != 'cuda':
+        raise TypeError(f'{name} must be a device (cuda) tensor')
+    want = torch.complex128 if dtype is None else dtype
+    if x.dtype != want:
+        raise TypeError(f'{name} must be {want}, not {x.dtype}')
+    if not x.is_contiguous():
+        raise ValueError(f'{name} must be contiguous')
+    if x.ndim < 2:
+        raise ValueError(f'{name} must have a band axis and grid axes')
+
+
 def apply_hamiltonian(kin, vt, psi, out, phase_cd=None, eps=None):
     """out = kin(psi) + vt psi (vt: real grid, float64 device tensor); with
     eps (float64 device tensor, one value per band) the residuals
     out_n = (kin + vt - eps_n) psi_n in the same pass
     (gpaw/new/pwfd/eigensolver.py:197-227)."""
+    want = torch.complex128 if kin.dtype == complex else torch.float64
+    _bands(psi, 'psi', want)
+    _bands(out, 'out', want)
     if tuple(psi.shape[-3:]) != kin.shape or tuple(vt.shape) != kin.shape:
         raise ValueError('grid shape mismatch')
-    if psi.shape != out.shape or not psi.is_contiguous() or \
-            not out.is_contiguous():
-        raise ValueError('psi / out: equal, contiguous shapes')
+    if psi.shape != out.shape:
+        raise ValueError('psi and out must have equal shapes')
     if vt.dtype != torch.float64 or not vt.is_contiguous():
         raise TypeError('vt must be a contiguous float64 tensor')
     keep, ph = (None, None)
@@ -60,13 +78,32 @@ def apply_hamiltonian(kin, vt, psi, out, phase_cd=None, eps=None):
     return out
 
 
+def _square(m, nb):
+    if m.dtype != torch.complex128 or not m.is_contiguous() or \
+            tuple(m.shape) != (nb, nb) or m.device.type != 'cuda':
+        raise ValueError(f'matrix: contiguous complex128 ({nb}, {nb}) '
+                         'device tensor')
+
+
+def _bands_vector(v, nb):
+    if v.dtype != torch.float64 or not v.is_contiguous() or \
+            v.numel() != nb or v.device.type != 'cuda':
+        raise ValueError(f'out: contiguous float64 device tensor of {nb}')
+
+
 def band_dots(a, b, dv, out=None):
     """out[n] = dv Re <a_n|b_n> for (nbands, ...) float64 / complex128
     device tensors."""
+    if a.dtype not in (torch.float64, torch.complex128):
+        raise TypeError('band_dots: float64 or complex128 bands')
+    _bands(a, 'a', a.dtype)
+    _bands(b, 'b', a.dtype)
     nb = a.shape[0]
     npts = _npts(a)
-    if b.shape != a.shape or b.dtype != a.dtype:
-        raise ValueError('a and b: same shape and dtype')
+    if b.shape != a.shape:
+        raise ValueError('a and b: same shape')
+    if out is not None:
+        _bands_vector(out, nb)
     if out is None:
         out = torch.empty(nb, dtype=torch.float64, device=a.device)
     _lib.check(_lib.load().gfd_band_dots(
@@ -78,7 +115,10 @@ def band_dots(a, b, dv, out=None):
 def overlap(psi, dv, lower_only=True, out=None):
     """S[i, j] = dv sum_x psi_i(x) conj(psi_j(x)) (lower tiles only when
     lower_only: the part invcholesky reads)."""
+    _bands(psi, 'psi')
     nb = psi.shape[0]
+    if out is not None:
+        _square(out, nb)
     if out is None:
         out = torch.zeros((nb, nb), dtype=torch.complex128,
                           device=psi.device)
@@ -90,7 +130,13 @@ def overlap(psi, dv, lower_only=True, out=None):
 
 def subspace_matrix(psi, hpsi, dv, out=None):
     """H[i, j] = dv sum_x conj(psi_i(x)) hpsi_j(x)."""
+    _bands(psi, 'psi')
+    _bands(hpsi, 'hpsi')
+    if hpsi.shape != psi.shape:
+        raise ValueError('psi and hpsi must have equal shapes')
     nb = psi.shape[0]
+    if out is not None:
+        _square(out, nb)
     if out is None:
         out = torch.empty((nb, nb), dtype=torch.complex128,
                           device=psi.device)
@@ -103,10 +149,14 @@ def subspace_matrix(psi, hpsi, dv, out=None):
 def orthonormalize(psi, dv, out=None, S=None):
     """psi <- C psi with C = chol(S)^-1, S = overlap(psi) (lower);
     returns (out, C)."""
+    _bands(psi, 'psi')
     nb = psi.shape[0]
     S = overlap(psi, dv, True, S)
     if out is None:
         out = torch.empty_like(psi)
+    _bands(out, 'out')
+    if out.shape != psi.shape or out.data_ptr() == psi.data_ptr():
+        raise ValueError('out: the shape of psi, not psi itself')
     _lib.check(_lib.load().gfd_band_cholesky_orth(
         ptr(S), nb, ptr(psi), _npts(psi), _npts(psi), ptr(out),
         stream_handle()), 'orthonormalize')

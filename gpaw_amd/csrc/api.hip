@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstdarg>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -378,18 +380,26 @@ extern "C" int gfd_operator_relax(gfd_operator* op, int method, double* f,
     return OK;
 }
 
+// partial: caller-owned partial-sum buffer of at least
+// op_max_blocks(op, dtype, nin) + 1 doubles, or NULL for the operator's own
+// (grown here; a caller that captures the pass into a graph passes its own,
+// sized beforehand, so no later call can reallocate it under the graph).
 static int residual_norm(gfd_operator* op, int dtype, const void* in,
                          const void* b, void* r, long nin,
                          const double* phases, double* dev_norm,
-                         hipStream_t s, bool narrow = false)
+                         hipStream_t s, bool narrow = false,
+                         double* partial = nullptr)
 {
     EpiArgs ea{};
     ea.aux = b;
     ea.c0 = centre_coef(op);
     if (dev_norm) {
-        const long nbmax = op_max_blocks(op, dtype, nin);
-        GFD_TRY(op->partial.ensure(sizeof(double) * (nbmax + 1), s));
-        ea.partial = (double*)op->partial.p;
+        if (!partial) {
+            const long nbmax = op_max_blocks(op, dtype, nin);
+            GFD_TRY(op->partial.ensure(sizeof(double) * (nbmax + 1), s));
+            partial = (double*)op->partial.p;
+        }
+        ea.partial = partial;
     }
     int nb = 0;
     GFD_TRY(op_pass(op, dtype, in, r, nin, phases,
@@ -511,17 +521,25 @@ extern "C" void gfd_transformer_free(gfd_transformer* t)
     delete t;
 }
 
-static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
-                   long nin, const double* phases, const XEpiArgs& ea,
-                   hipStream_t s)
+static size_t xfer_scratch_bytes(const gfd_transformer* t, int dtype,
+                                 long nin)
 {
     const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);  // 3: c64
-    if (t->plan.K != 2)
-        GFD_TRY(t->scratch.ensure(es * xfer_scratch_elems(t->plan, nin) + 16,
-                                  s));
+    return t->plan.K == 2 ? 0 : es * xfer_scratch_elems(t->plan, nin) + 16;
+}
+
+// work: caller-owned work space of xfer_scratch_bytes() (graph-captured
+// drivers), or NULL for the transformer's own.
+static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
+                   long nin, const double* phases, const XEpiArgs& ea,
+                   hipStream_t s, void* work = nullptr)
+{
+    if (!work && t->plan.K != 2) {
+        GFD_TRY(t->scratch.ensure(xfer_scratch_bytes(t, dtype, nin), s));
+        work = t->scratch.p;
+    }
     const BC bc = make_bc(t->periodic, t->real ? nullptr : phases);
-    return launch_transfer(t->plan, dtype, bc, in, out, nin, t->scratch.p, ea,
-                           s);
+    return launch_transfer(t->plan, dtype, bc, in, out, nin, work, ea, s);
 }
 
 static int xfer(gfd_transformer* t, const void* in, void* out, long nin,
@@ -724,6 +742,11 @@ struct gfd_poisson {
     Levels lv;     // f64 levels (phi[0], rho[0]: owned copies for graphs)
     Levels lv32;   // f32 levels of the mixed-precision inner cycles
     DevBuf scalar; // norms
+    // partial sums of the norms and work space of K > 2 transfers: owned
+    // here (not borrowed from the level operators / transformers, which
+    // other callers may regrow) because captured graphs hold their
+    // addresses; sized once in gfd_poisson_new
+    DevBuf partial, xscratch;
     // graph replay of one V-cycle (+ residual norm) on an owned stream
     bool use_graph = true;
     hipStream_t gs = nullptr;
@@ -807,8 +830,20 @@ extern "C" int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
     p->w.assign(weights, weights + nlevels + 1);
     p->dv = dv;
     int st = alloc_levels(p, p->lv, 0, nlevels);
+    // every norm of the drivers is over level 0 (one grid); transfers of
+    // either precision use at most the FP64 scratch size
+    const long ng0 = p->ops[0]->ng();
+    const long np = std::max(op_max_blocks(p->ops[0], 0, 1),
+                             vec_dot_partials(ng0)) + 1;
+    if (st == OK) st = p->partial.ensure(sizeof(double) * np, nullptr);
+    size_t xb = 16;
+    for (int l = 0; l < nlevels; l++)
+        xb = std::max({xb, xfer_scratch_bytes(p->rst[l], 0, 1),
+                       xfer_scratch_bytes(p->itp[l], 0, 1)});
+    if (st == OK) st = p->xscratch.ensure(xb, nullptr);
+    if (st == OK) st = p->scalar.ensure(4 * sizeof(double), nullptr);
     if (st != OK) {
-        delete p;
+        gfd_poisson_free(p);
         return st;
     }
     *out = p;
@@ -833,6 +868,8 @@ extern "C" void gfd_poisson_free(gfd_poisson* p)
     p->lv.pool.release();
     p->lv32.pool.release();
     p->scalar.release();
+    p->partial.release();
+    p->xscratch.release();
     delete p;
 }
 
@@ -987,7 +1024,7 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
                         nullptr));
         XEpiArgs xe;
         GFD_TRY(xfer_dt(p->rst[l], dt, lv.res[l], lv.rho[l + 1], 1, nullptr,
-                        xe, s));
+                        xe, s, p->xscratch.p));
         void* c1 = lv.phi[l + 1];
         void* a1 = lv.tmp[l + 1];
         const CoarseSolve& cs = dt == 2 ? p->cs32 : p->cs64;
@@ -1002,7 +1039,8 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
         XEpiArgs ce;
         ce.epi = XEPI_SUBFROM;
         ce.base = *cur;
-        GFD_TRY(xfer_dt(p->itp[l], dt, c1, *alt, 1, nullptr, ce, s));
+        GFD_TRY(xfer_dt(p->itp[l], dt, c1, *alt, 1, nullptr, ce, s,
+                        p->xscratch.p));
         std::swap(*cur, *alt);
     }
     GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_post(sc, l),
@@ -1033,7 +1071,7 @@ static int vcycle64_norm(gfd_poisson* p, double* phi, const double* rho,
     if (cur != (void*)phi)
         GFD_TRY(vec_copy((const double*)cur, phi, p->ops[0]->ng(), s));
     GFD_TRY(residual_norm(p->ops[0], 0, phi, rho, p->lv.res[0], 1, nullptr,
-                          dn, s));
+                          dn, s, false, (double*)p->partial.p));
     return OK;
 }
 
@@ -1067,9 +1105,6 @@ static int pretune_levels(gfd_poisson* p, Levels& lv, hipStream_t s)
 template <typename F>
 static int capture(gfd_poisson* p, hipGraphExec_t* exec, F fn)
 {
-    gfd_operator* op0 = p->ops[0];
-    GFD_TRY(op0->partial.ensure(
-        sizeof(double) * (op_max_blocks(op0, 0, 1) + 1), p->gs));
     GFD_TRY(pretune_levels(p, p->lv, p->gs));
     if (p->lv32.L >= 0) GFD_TRY(pretune_levels(p, p->lv32, p->gs));
     hipGraph_t g = nullptr;
@@ -1130,10 +1165,8 @@ extern "C" int gfd_add_scalar(double* a, long n, double v, void* stream);
 static int remove_mean(gfd_poisson* p, double* phi, hipStream_t s)
 {
     const long ng = p->ops[0]->ng();
-    GFD_TRY(p->ops[0]->partial.ensure(
-        sizeof(double) * (vec_dot_partials(ng) + 1), s));
     double* dn = (double*)p->scalar.p;
-    GFD_TRY(vec_dot(phi, nullptr, ng, (double*)p->ops[0]->partial.p, dn, s));
+    GFD_TRY(vec_dot(phi, nullptr, ng, (double*)p->partial.p, dn, s));
     double sum = 0.0;
     GFD_TRY(read_scalar(p, s, &sum));
     const double ave = sum / (double)ng;
@@ -1248,7 +1281,7 @@ static int mixed_step(gfd_poisson* p, const Schedule& sc, int inner,
         GFD_TRY(vcycle(p, lv, sc, 0, &cur, &alt, r32, s));
     GFD_TRY(vec_sub_f(phi0, (const float*)cur, ng, s));
     return residual_norm(p->ops[0], 0, phi0, rho0, r32, 1, nullptr, dn, s,
-                         true);
+                         true, (double*)p->partial.p);
 }
 
 extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
@@ -1292,12 +1325,10 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
     const double* rho0 = (const double*)p->lv.rho[0];
     // |rho|^2 and the first residual
     double rn2 = 0.0, err2 = 0.0;
-    GFD_TRY(p->ops[0]->partial.ensure(
-        sizeof(double) * (vec_dot_partials(ng) + 1), p->gs));
-    GFD_TRY(vec_dot(rho0, rho0, ng, (double*)p->ops[0]->partial.p, dn, p->gs));
+    GFD_TRY(vec_dot(rho0, rho0, ng, (double*)p->partial.p, dn, p->gs));
     GFD_TRY(read_scalar(p, p->gs, &rn2));
     GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, p->lv32.rho[0], 1,
-                          nullptr, dn, p->gs, true));
+                          nullptr, dn, p->gs, true, (double*)p->partial.p));
     GFD_TRY(read_scalar(p, p->gs, &err2));
     const double tol2 = tol_rel * tol_rel * rn2;
     int outer = 0;
@@ -1332,7 +1363,28 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
 
 // =================================================================== bands
 
-static BandWork g_band;
+// Work spaces of the band calls and of gfd_dot, one per stream, so that
+// calls on different streams never share partial sums.  They live until the
+// process ends (the HIP runtime may be gone at static teardown).
+static std::mutex g_ws_mu;
+
+static BandWork& band_work(hipStream_t s)
+{
+    static std::map<hipStream_t, BandWork*> m;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    BandWork*& w = m[s];
+    if (!w) w = new BandWork();
+    return *w;
+}
+
+static DevBuf& dot_work(hipStream_t s)
+{
+    static std::map<hipStream_t, DevBuf*> m;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    DevBuf*& w = m[s];
+    if (!w) w = new DevBuf();
+    return *w;
+}
 
 extern "C" int gfd_h_apply(gfd_operator* kin, const double* vpot,
                            const void* psi, void* hpsi, long nbands,
@@ -1419,7 +1471,8 @@ extern "C" int gfd_band_overlap(const void* psi, int nbands, long npts,
         set_error("gfd_band_overlap: bad arguments");
         return EINVAL_;
     }
-    return gram(g_band, (const double2*)psi, ld, (const double2*)psi, ld,
+    return gram(band_work((hipStream_t)stream), (const double2*)psi, ld,
+                (const double2*)psi, ld,
                 nbands, nbands, npts, dv, 0, lower_only, (double2*)S, nbands,
                 (hipStream_t)stream);
 }
@@ -1432,7 +1485,8 @@ extern "C" int gfd_band_matrix(const void* psi, const void* hpsi, int nbands,
         set_error("gfd_band_matrix: bad arguments");
         return EINVAL_;
     }
-    return gram(g_band, (const double2*)psi, ld, (const double2*)hpsi, ld,
+    return gram(band_work((hipStream_t)stream), (const double2*)psi, ld,
+                (const double2*)hpsi, ld,
                 nbands, nbands, npts, dv, 1, 0, (double2*)H, nbands,
                 (hipStream_t)stream);
 }
@@ -1446,13 +1500,12 @@ extern "C" int gfd_band_cholesky_orth(void* S, int nbands, const void* psi,
         set_error("gfd_band_cholesky_orth: bad arguments");
         return EINVAL_;
     }
-    return cholesky_orth(g_band, (double2*)S, nbands, (const double2*)psi, ld,
+    return cholesky_orth(band_work((hipStream_t)stream), (double2*)S, nbands,
+                         (const double2*)psi, ld,
                          npts, (double2*)out, ld, (hipStream_t)stream);
 }
 
 // ==================================================================== misc
-
-static DevBuf g_dot_buf;
 
 extern "C" int gfd_dot(const double* a, const double* b, long n,
                        double* result, void* stream)
@@ -1462,8 +1515,9 @@ extern "C" int gfd_dot(const double* a, const double* b, long n,
         return EINVAL_;
     }
     hipStream_t s = (hipStream_t)stream;
-    GFD_TRY(g_dot_buf.ensure(sizeof(double) * (vec_dot_partials(n) + 2), s));
-    double* part = (double*)g_dot_buf.p;
+    DevBuf& buf = dot_work(s);
+    GFD_TRY(buf.ensure(sizeof(double) * (vec_dot_partials(n) + 2), s));
+    double* part = (double*)buf.p;
     double* dres = part + vec_dot_partials(n) + 1;
     GFD_TRY(vec_dot(a, b, n, part, dres, s));
     GFD_CHECK_HIP(hipMemcpyAsync(result, dres, sizeof(double),
@@ -1490,6 +1544,31 @@ extern "C" int gfd_stream_copy(const void* src, void* dst, long nbytes,
         return EINVAL_;
     }
     return vec_copy16(src, dst, nbytes, (hipStream_t)stream);
+}
+
+static long tile_code(const XmTile& t)
+{
+    return (long)t.tz | (long)t.nseg << 12 | (long)t.py << 24 |
+           (long)t.xchunk << 32 | (long)t.dma << 48 | (long)t.order << 52 |
+           (long)t.wide << 56;
+}
+
+extern "C" int gfd_row_tile_candidates(int radius, int dtype,
+                                       const long n[3], int with_dma,
+                                       long* codes, int max_codes)
+{
+    if (radius < 1 || radius > MAXR_FAST || dtype < 0 || dtype > 3 || !n ||
+        n[0] < 1 || n[1] < 1 || n[2] < 1 || max_codes < 0 ||
+        (max_codes > 0 && !codes)) {
+        set_error("gfd_row_tile_candidates: bad arguments");
+        return -1;
+    }
+    const int nn[3] = {(int)n[0], (int)n[1], (int)n[2]};
+    const std::vector<XmTile> c = xr_candidates(radius, dtype, nn,
+                                                with_dma != 0);
+    for (int i = 0; i < (int)c.size() && i < max_codes; i++)
+        codes[i] = tile_code(c[i]);
+    return (int)c.size();
 }
 
 extern "C" int gfd_set_tuning(const char* key, long value)

@@ -6,8 +6,9 @@
 namespace gfd {
 
 // Grow-only device work space of the band calls.  It lives as long as the
-// process (no destructor: the HIP runtime may be gone at static teardown),
-// and calls that overlap on different streams race on it.
+// process (no destructor: the HIP runtime may be gone at static teardown).
+// The C ABI keeps one per stream, so only calls queued on one stream share
+// it, and those run in order.
 struct BandWork {
     void* partial = nullptr;
     size_t partial_bytes = 0;

@@ -2,6 +2,8 @@
 // fused residual forms used by the multigrid drivers).
 #pragma once
 
+#include <vector>
+
 #include "gfd_common.h"
 
 namespace gfd {
@@ -86,6 +88,9 @@ bool row_stencil_fits(const AxisStencil& st, int dtype, const int n[3],
 long row_stencil_max_blocks(const AxisStencil& st, int dtype, const int n[3],
                             long nin);
 XmTile xr_default_tile(int R, int dtype, const int n[3], long nin);
+// The tiles the tuner times for a radius, data type and grid shape besides
+// the heuristic one (k_xrow tiles, then with dma the k_xdma tiles).
+std::vector<XmTile> xr_candidates(int R, int dtype, const int n[3], bool dma);
 // Timing autotuner of the full-row kernel's tile (on by default; off also
 // forgets the tiles found).
 void xr_set_autotune(bool on);

@@ -202,19 +202,14 @@ static int row_launch(const AxisStencil& st, const XmTile& tl, int dtype,
     return ENOTSUP_;
 }
 
-static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
-                       const BC& bc, const void* in, void* out, long nin,
-                       hipStream_t stream, const XmTile& heur, bool dma)
+// Every tile the tuner times for a radius, type and shape (after the
+// heuristic tile, which it also times): k_xrow tiles, then (dma) k_xdma
+// tiles.
+std::vector<XmTile> xr_candidates(int R, int dtype, const int n[3], bool dma)
 {
-    const int R = st.R;
     const int PZ = xr_pz(dtype);
     const int TZT = std::max(1, n[2] / PZ);
-    // time on at most ~32M points of the batch (the ranking of tiles
-    // does not change beyond a few waves per CU)
-    const long per = (long)n[0] * n[1] * n[2];
-    const long nt_in = std::max(1L, std::min(nin, (32L << 20) / per));
     std::vector<XmTile> cand;
-    cand.push_back(heur);
     const int pymax = R >= 3 ? 2 : 4;
     static const int xcs[] = {4, 8, 12, 16, 24, 32, 48, 64};
     for (int py = 1; py <= pymax; py <<= 1)
@@ -285,6 +280,21 @@ static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
                         }
             }
     }
+    return cand;
+}
+
+static XrTuned xr_tune(const AxisStencil& st, int dtype, const int n[3],
+                       const BC& bc, const void* in, void* out, long nin,
+                       hipStream_t stream, const XmTile& heur, bool dma)
+{
+    const int R = st.R;
+    // time on at most ~32M points of the batch (the ranking of tiles
+    // does not change beyond a few waves per CU)
+    const long per = (long)n[0] * n[1] * n[2];
+    const long nt_in = std::max(1L, std::min(nin, (32L << 20) / per));
+    std::vector<XmTile> cand;
+    cand.push_back(heur);
+    for (const XmTile& t : xr_candidates(R, dtype, n, dma)) cand.push_back(t);
     XrTuned res;
     res.row = heur;
     res.dma = heur;
@@ -376,8 +386,12 @@ static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
         (void)hipGetLastError();
         return heur;
     }
+    // candidates are timed as PLAIN passes of the call's dtype into `out`:
+    // not where out holds another element size (RESN writes FP32 from FP64
+    // input, UPDW FP64 from FP32 compute) or aliases an input
     if (cs != hipStreamCaptureStatusNone || in == out ||
-        (ea.aux && ea.aux == out) || (ea.pot && (const void*)ea.pot == out))
+        (ea.aux && ea.aux == out) || (ea.pot && (const void*)ea.pot == out) ||
+        epi == EPI_RESN || epi == EPI_UPDW)
         return heur;
     const XrTuned t =
         xr_tune(st, dtype, n, bc, in, out, nin, stream, heur, dma);

@@ -3,7 +3,7 @@
 // Bloch-phase boundaries.
 //
 // A 3-D transfer is the product of three 1-D transfers (z, then y, then x,
-// the order of c/bmgs/restrict.cpp:153-155 and interpolate.cpp:245-256).
+// the order of c/bmgs/restrict.cpp:153-155 and interpolate.cpp:86-97).
 // Each pass is one kernel with one thread per output point; the boundary of
 // the pass's axis is taken by index wrap, the other two axes are in range.
 //

@@ -1,17 +1,22 @@
 """Finite-difference operators on the GPU (drop-in for gpaw.fd_operators).
 
-Same classes, signatures and coefficient construction as the reference
-(gpaw/fd_operators.py); the stencil itself runs in libgpaw_fd.so
-(include/gpaw_fd.h) on torch-allocated device memory.
+The classes keep the reference's names, signatures and attributes
+(gpaw/fd_operators.py:42-396); the stencil runs in libgpaw_fd.so
+(include/gpaw_fd.h) on torch-allocated device memory.  The coefficients are
+built here from the defining equations, written down next to each builder:
 
-- ``laplace`` / ``derivatives`` tables: gpaw/fd_operators.py:27-39
-  (Chelikowsky et al., PRB 50, 11355 (1994)); the tables have to be the same
-  numbers to give the same operator.
-- ``FDOperator``: gpaw/fd_operators.py:42-147
-- ``GUCLaplace`` / ``Laplace``: :177-238
-- ``find_neighbors`` / ``Gradient``: :241-354 (the Minkowski reduction that
-  the reference takes from ASE is done by ``minkowski_reduce`` below)
-- ``LaplaceA`` / ``LaplaceB`` (Mehrstellen): :357-396
+- ``laplace`` / ``derivatives``: the central-difference weights of
+  Chelikowsky et al., PRB 50, 11355 (1994), as tabulated in
+  gpaw/fd_operators.py:27-39 (the same numbers give the same operator).
+- Laplacian on any cell (``GUCLaplace``): sum_d a_d (h_d . grad)^2 with
+  sum_d a_d h_d h_d^T = 1, over the fewest of the shortest lattice steps h_d
+  for which that tensor equation has an exact solution.  Orthorhombic cells
+  have the closed form a_d = 1 / h_d^2 along the three axes.
+- Gradient on any cell: the derivative along the unit normal n_c of each
+  cell plane as a minimum-norm combination of 1-D derivatives along the
+  Voronoi-relevant lattice steps, recombined into the Cartesian direction v.
+- Mehrstellen pair ``LaplaceA`` / ``LaplaceB``: the O(h^4) compact scheme
+  A phi = B f for lap phi = f (Collatz), coefficients derived below.
 """
 from __future__ import annotations
 
@@ -24,6 +29,8 @@ from . import _lib
 from .arrays import (Staging, check_contiguous, dtype_of, is_device, ptr,
                      stream_handle)
 
+# Central second-derivative weights: laplace[n] = [w_0, w_1, .., w_n] with
+# h^2 f''(x) ~ w_0 f(x) + sum_k w_k (f(x + k h) + f(x - k h)).
 laplace = [[0],
            [-2, 1],
            [-5 / 2, 4 / 3, -1 / 12],
@@ -33,14 +40,33 @@ laplace = [[0],
            [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925,
             -1 / 16632]]
 
+# Central first-derivative weights: derivatives[n - 1] = [d_1, .., d_n] with
+# h f'(x) ~ sum_k d_k (f(x + k h) - f(x - k h)).
 derivatives = [[1 / 2],
                [2 / 3, -1 / 12],
                [3 / 4, -3 / 20, 1 / 60],
                [4 / 5, -1 / 5, 4 / 105, -1 / 280]]
 
+# The 13 lattice steps (i, j, k), |i|, |j|, |k| <= 1, whose first nonzero
+# component is positive, in lexicographic order; -m is the other half.
+_HALF_STEPS = np.array([m for m in itertools.product((-1, 0, 1), repeat=3)
+                        if any(m) and m[np.flatnonzero(m)[0]] > 0], int)
+
 
 def _nin(shape):
     return int(np.prod(shape[:-3], dtype=np.int64))
+
+
+def _line(step_c, weights, sign):
+    """Offsets and coefficients of one 1-D stencil along step_c: the points
+    k step_c for k = 1..n with `weights`, then k = -1..-n with
+    sign * weights."""
+    n = len(weights)
+    step_c = np.asarray(step_c, int)
+    offs = [k * step_c for k in range(1, n + 1)]
+    offs += [-k * step_c for k in range(1, n + 1)]
+    coefs = list(weights) + [sign * w for w in weights]
+    return offs, coefs
 
 
 class FDOperator:
@@ -48,46 +74,44 @@ class FDOperator:
 
     ``apply`` / ``__call__`` / ``relax`` accept device tensors (in place,
     asynchronous on torch's current stream) or numpy arrays (staged through
-    device memory, synchronous).
+    device memory, synchronous).  Attributes as the reference's:
+    ``coef_p``, ``offset_pc``, ``offset_p`` (offsets flattened with the
+    strides of the grid padded by ``mp`` on every side), ``mp`` (stencil
+    radius), ``cfd`` (every point on an axis through the centre),
+    ``npoints``, ``description``.
     """
 
     def __init__(self, coef_p, offset_pc, gd, dtype=float,
                  description=None, xp=np):
-        offset_pc = np.asarray(offset_pc, dtype=int).reshape((-1, 3))
-        cfd = True
-        for offset_c in offset_pc:
-            if sum([offset != 0 for offset in offset_c]) >= 2:
-                cfd = False
-        mp = np.abs(offset_pc).max()
-        assert (mp <= gd.n_c).all(), 'Stencil longer than domain'
-        n_c = gd.n_c
-        M_c = n_c + 2 * mp
-        stride_c = np.array([M_c[1] * M_c[2], M_c[2], 1])
-        offset_p = np.dot(offset_pc, stride_c)
-        coef_p = np.ascontiguousarray(coef_p, float)
-        neighbor_cd = gd.neighbor_cd
-        assert coef_p.ndim == 1
-        assert coef_p.shape == offset_p.shape
-        assert dtype in [float, complex]
+        if dtype not in (float, complex):
+            raise TypeError(f'dtype must be float or complex, not {dtype}')
+        offsets = np.asarray(offset_pc, dtype=int).reshape((-1, 3))
+        coefs = np.ascontiguousarray(np.asarray(coef_p, float))
+        if coefs.ndim != 1 or len(coefs) != len(offsets):
+            raise ValueError('one coefficient per offset')
+        radius = int(np.abs(offsets).max()) if len(offsets) else 0
+        if (radius > np.asarray(gd.n_c)).any():
+            # the reference asserts this (gpaw/fd_operators.py:53)
+            raise AssertionError('Stencil longer than domain')
+        padded = np.asarray(gd.n_c, int) + 2 * radius
+        strides = np.array([padded[1] * padded[2], padded[2], 1])
         self.dtype = dtype
-        self.shape = tuple(int(n) for n in n_c)
-        self.comm = None
-        self.mp = mp
         self.gd = gd
-        self.npoints = len(coef_p)
-        self.coef_p = coef_p
-        self.offset_p = offset_p
-        self.offset_pc = offset_pc
-        self.cfd = cfd
+        self.shape = tuple(int(x) for x in gd.n_c)
+        self.comm = None
         self.xp = xp
-
+        self.coef_p = coefs
+        self.offset_pc = offsets
+        self.offset_p = offsets @ strides
+        self.npoints = len(coefs)
+        self.mp = radius
+        self.cfd = bool((np.count_nonzero(offsets, axis=1) <= 1).all())
+        self.description = (description if description is not None else
+                            f'{self.npoints} point finite-difference '
+                            'stencil')
+        self.operator = self  # the reference exposes its C object here
         self._h = None
         self._lib = None
-        self.operator = self  # reference attribute: the C object
-
-        if description is None:
-            description = '%d point finite-difference stencil' % self.npoints
-        self.description = description
 
     def _handle(self):
         """The libgpaw_fd operator (created on first use, needs the GPU)."""
@@ -115,7 +139,9 @@ class FDOperator:
     def __str__(self):
         return '<' + self.description + '>'
 
-    # -- argument checks of the GPAW_DEBUG wrapper (fd_operators.py:150-174)
+    # -- the argument checks the reference makes in debug mode
+    #    (gpaw/fd_operators.py:150-174), always on here: the kernels trust
+    #    shapes and dtypes
     def _check(self, in_xg, out_xg, phase_cd):
         if tuple(in_xg.shape) != tuple(out_xg.shape):
             raise ValueError('input and output shapes differ')
@@ -147,7 +173,7 @@ class FDOperator:
         st.download(1)
 
     def __call__(self, in_xR, out_xR=None):
-        """New UGArray interface (fd_operators.py:102-118)."""
+        """Array-object interface (gpaw/fd_operators.py:102-118)."""
         if out_xR is None:
             out_xR = in_xR.new()
         phase = getattr(in_xR.desc, 'phase_factor_cd', None)
@@ -155,7 +181,8 @@ class FDOperator:
         return out_xR
 
     def relax(self, relax_method, f_g, s_g, n, w=None):
-        """n sweeps of relax_method (2: weighted Jacobi) for self(f) = s."""
+        """n sweeps of relax_method (2: weighted Jacobi, 1: Gauss-Seidel)
+        for self(f) = s."""
         if self.dtype != float:
             raise TypeError('relax: real operators only')
         if tuple(f_g.shape) != self.shape or tuple(s_g.shape) != self.shape:
@@ -175,6 +202,8 @@ class FDOperator:
 
     def residual(self, in_xg, b_xg, r_xg, phase_cd=None, norm=False):
         """r = self(in) - b, and sum |r|^2 when ``norm`` (fused)."""
+        self._check(in_xg, r_xg, phase_cd)
+        self._check(b_xg, r_xg, phase_cd)
         st = Staging(in_xg, b_xg, r_xg)
         keep, ph = self._phases(phase_cd)
         out = _lib.ctypes.c_double(0.0)
@@ -188,18 +217,61 @@ class FDOperator:
         return out.value if norm else None
 
     def get_diagonal_element(self):
-        """Coefficient of the (last) zero offset (c/operators.cpp:319-332)."""
-        d = 0.0
-        for c, o in zip(self.coef_p, self.offset_p):
-            if o == 0:
-                d = float(c)
-        return d
+        """Coefficient of the zero offset, the last one if it repeats (as
+        c/operators.cpp:319-332 reads it)."""
+        centre = np.flatnonzero(self.offset_p == 0)
+        return float(self.coef_p[centre[-1]]) if len(centre) else 0.0
 
     def get_async_sizes(self):
         return (0, 1, 1)
 
 
+# ------------------------------------------------------------- Laplacian
+
+def _second_moment_rows(h_dv):
+    """The six independent entries (xx, yy, zz, yz, xz, xy) of h_d h_d^T,
+    one column per step d."""
+    x, y, z = np.asarray(h_dv, float).T
+    return np.array([x * x, y * y, z * z, y * z, x * z, x * y])
+
+
+_UNIT_TENSOR = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+
+
+def laplacian_directions(h_cv):
+    """(M_dc, a_d): lattice steps M_dc (grid-index units) and weights with
+    sum_d a_d (M_d h)(M_d h)^T = 1, so that lap = sum_d a_d (h_d . grad)^2
+    with h_d = M_d h.
+
+    Candidate steps are the 13 half-space steps ordered by their length in
+    units of the cell-vector lengths (ties in lexicographic order, so an
+    orthorhombic cell gives z, y, x); the first D = 3, 4, 5, 6 of them
+    whose tensor equation is solvable are used.  A rank-deficient set of
+    steps means the cell needs reducing first (the reference raises the
+    same ValueError, gpaw/fd_operators.py:216-220)."""
+    h_cv = np.asarray(h_cv, float)
+    if np.count_nonzero(h_cv - np.diag(np.diag(h_cv))) == 0:
+        h = np.diag(h_cv)
+        return np.array([(0, 0, 1), (0, 1, 0), (1, 0, 0)]), \
+            1.0 / h[::-1]**2
+    unit_cv = h_cv / np.linalg.norm(h_cv, axis=1)[:, None]
+    length = ((_HALF_STEPS @ unit_cv)**2).sum(1)
+    order = np.argsort(length, kind='stable')
+    for D in range(3, 7):
+        M_dc = _HALF_STEPS[order[:D]]
+        A = _second_moment_rows(M_dc @ h_cv)          # 6 x D
+        if np.linalg.matrix_rank(A) < D:
+            raise ValueError('You have a weird unit cell!  Try to use the '
+                             'maximally reduced Niggli cell.')
+        Q, R = np.linalg.qr(A)
+        a_d = np.linalg.solve(R, Q.T @ _UNIT_TENSOR)
+        if ((A @ a_d - _UNIT_TENSOR)**2).sum() < 1e-14 or D == 6:
+            return M_dc, a_d
+    raise AssertionError('unreachable')
+
+
 def Laplace(gd, scale=1.0, n=1, dtype=float, xp=np):
+    """scale * Laplacian with O(h^2n) error (gpaw/fd_operators.py:177-181)."""
     if n == 9:
         raise NotImplementedError('FTLaplace (n=9) is a host FFT operator, '
                                   'not part of the FD grid path')
@@ -208,46 +280,28 @@ def Laplace(gd, scale=1.0, n=1, dtype=float, xp=np):
 
 class GUCLaplace(FDOperator):
     def __init__(self, gd, scale=1.0, n=1, dtype=float, xp=np):
-        """Laplacian for general non orthorhombic grid.
+        """scale * Laplacian on any cell; scale=-0.5 is the kinetic energy
+        operator (gpaw/fd_operators.py:184-238).
 
-        scale=-0.5 gives the kinetic energy operator; the stencil has
-        O(h^(2n)) error (gpaw/fd_operators.py:184-238).
+        Stencil layout: the centre first, then for each direction d the
+        points +1..+n and -1..-n along M_d.
         """
-        # the 13 inequivalent neighbour directions, shortest first
-        M_ic = np.indices((3, 3, 3)).reshape((3, -3)).T[-13:] - 1
-        u_cv = gd.h_cv / (gd.h_cv**2).sum(1)[:, np.newaxis]**0.5
-        u2_i = (np.dot(M_ic, u_cv)**2).sum(1)
-        i_d = u2_i.argsort()
-
-        m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
-                         (0, 1, 1), (1, 0, 1), (1, 1, 0)])
-        for D in range(3, 7):
-            h_dv = np.dot(M_ic[i_d[:D]], gd.h_cv)
-            A_md = (h_dv**m_mv[:, np.newaxis, :]).prod(2)
-            a_d, residual, rank, s = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
-                                                     rcond=-1)
-            if residual.sum() < 1e-14:
-                if rank != D:
-                    raise ValueError(
-                        'You have a weird unit cell!  '
-                        'Try to use the maximally reduced Niggli cell.')
-                break
-
-        a_d *= scale
-        offsets = [(0, 0, 0)]
-        coefs = [laplace[n][0] * a_d.sum()]
-        for d in range(D):
-            M_c = M_ic[i_d[d]]
-            offsets.extend(np.arange(1, n + 1)[:, np.newaxis] * M_c)
-            coefs.extend(a_d[d] * np.array(laplace[n][1:]))
-            offsets.extend(np.arange(-1, -n - 1, -1)[:, np.newaxis] * M_c)
-            coefs.extend(a_d[d] * np.array(laplace[n][1:]))
-
+        M_dc, a_d = laplacian_directions(gd.h_cv)
+        a_d = scale * np.asarray(a_d)
+        w = np.array(laplace[n], float)
+        offsets = [np.zeros(3, int)]
+        coefs = [w[0] * a_d.sum()]
+        for M_c, a in zip(M_dc, a_d):
+            o, c = _line(M_c, a * w[1:], +1)
+            offsets += o
+            coefs += c
         super().__init__(coefs, offsets, gd, dtype, xp=xp)
-        self.description = (
-            '%d*%d+1=%d point O(h^%d) finite-difference Laplacian' %
-            ((self.npoints - 1) // n, n, self.npoints, 2 * n))
+        D = len(a_d)
+        self.description = (f'{D}*{n}+1={self.npoints} point O(h^{2 * n}) '
+                            'finite-difference Laplacian')
 
+
+# -------------------------------------------------------------- gradient
 
 def minkowski_reduce(h_cv):
     """Minkowski-reduced basis of the lattice spanned by the rows of h_cv.
@@ -296,110 +350,150 @@ def minkowski_reduce(h_cv):
     return B, U
 
 
+def voronoi_steps(h_bv, tol=1e-10):
+    """Voronoi-relevant lattice vectors of a reduced basis h_bv, one of each
+    +-pair (first nonzero coefficient positive), as basis coefficients,
+    with their squared lengths.
+
+    v is relevant (the bisector plane of 0 and v bounds the Voronoi cell
+    of 0 with a face of positive area) exactly when v / 2 is strictly closer
+    to 0 and v than to every other lattice point u: u . v < |u|^2.  For a
+    reduced 3-D basis the relevant vectors and the points that can cut
+    their faces are among the 26 steps with coefficients in {-1, 0, 1}.
+    """
+    h_bv = np.asarray(h_bv, float)
+    steps = np.array([m for m in itertools.product((-1, 0, 1), repeat=3)
+                      if any(m)], int)
+    vec = steps @ h_bv
+    norm2 = (vec**2).sum(1)
+    keep = []
+    for i, m in enumerate(steps):
+        if m[np.flatnonzero(m)[0]] < 0:
+            continue
+        others = np.ones(len(steps), bool)
+        others[i] = False
+        dots = vec[others] @ vec[i]
+        if (dots < norm2[others] * (1 - tol)).all():
+            keep.append(i)
+    return steps[keep], norm2[keep]
+
+
 def find_neighbors(h_cv):
-    """Nearest neighbours sorted after distance (gpaw/fd_operators.py:241-281).
+    """Grid steps to the Voronoi neighbours of a grid point, shortest first
+    (one of each +-pair), in units of the cell's grid vectors: the
+    neighbours are at ``find_neighbors(h_cv) @ h_cv``
+    (gpaw/fd_operators.py:241-281).
 
     >>> find_neighbors([[0.1, 0, 0], [0, 0.11, 0], [0, 0, 0.12]])
     array([[1, 0, 0],
            [0, 1, 0],
            [0, 0, 1]])
     """
-    from scipy.spatial import Voronoi
     h_bv, U_bc = minkowski_reduce(h_cv)
-    M_ib = np.indices((3, 3, 3)).reshape((3, -1)).T - 1
-    h_iv = M_ib.dot(h_bv)
-    voro = Voronoi(h_iv)
-    i_d = []
-    for i1, i2 in voro.ridge_points:
-        if i1 == 13 and i2 > 13:
-            i_d.append(i2)
-        elif i2 == 13 and i1 > 13:
-            i_d.append(i1)
-    h2_d = (h_iv[i_d]**2).sum(1)
-    M_db = M_ib[[i_d[d] for d in h2_d.argsort()]]
-    return M_db @ U_bc
+    M_db, len2 = voronoi_steps(h_bv)
+    order = np.argsort(np.round(len2, 12), kind='stable')
+    return M_db[order] @ U_bc
+
+
+def gradient_weights(h_cv, v):
+    """(M_dc, g_d): the derivative along Cartesian axis v as
+    sum_d g_d (h_d . grad), h_d = M_d h, over the Voronoi steps M_d.
+
+    For each cell axis c, the derivative along the unit normal n_c of the
+    plane of the other two cell vectors is built from the steps that leave
+    that plane: the minimum-norm g with sum_d g_d h_d = n_c, posed on the
+    components along (n_c, h_{c-1}, h_{c-2}) -- which selects the
+    reference's stencil among the many exact ones
+    (gpaw/fd_operators.py:310-334).  Then d/dv = sum_c (N^-1)_{vc} n_c . grad
+    with N the matrix whose rows are the normals.
+    """
+    h_cv = np.asarray(h_cv, float)
+    M_dc = find_neighbors(h_cv)
+    h_dv = M_dc @ h_cv
+    dual_vc = np.linalg.inv(h_cv)
+    normals = (dual_vc / np.linalg.norm(dual_vc, axis=0)).T   # rows n_c
+    g_cd = np.zeros((3, len(h_dv)))
+    for c, n_v in enumerate(normals):
+        frame = np.array([n_v, h_cv[(c + 2) % 3], h_cv[(c + 1) % 3]])
+        leaving = np.abs(h_dv @ n_v) > 1e-10
+        system = frame @ h_dv[leaving].T                       # 3 x m
+        g, *_ = np.linalg.lstsq(system, np.array([1.0, 0.0, 0.0]),
+                                rcond=None)
+        g_cd[c, leaving] = g
+    return M_dc, np.linalg.inv(normals)[v] @ g_cd
 
 
 class Gradient(FDOperator):
     def __init__(self, grid, v, scale=1.0, n=1, dtype=float, xp=np):
-        """Symmetric gradient along v (0, 1, 2) for any cell
-        (gpaw/fd_operators.py:284-354)."""
+        """scale * d/dr_v with O(h^2n) error on any cell
+        (gpaw/fd_operators.py:284-354).
+
+        Stencil layout: for each Voronoi step with a nonzero weight, the
+        points +1..+n then -1..-n; no centre point.
+        """
         gd = getattr(grid, '_gd', grid)
-        M_dc = find_neighbors(gd.h_cv)
-        h_dv = M_dc @ gd.h_cv
-        D = len(h_dv)
-        invh_vc = np.linalg.inv(gd.h_cv)
-        n_cv = (invh_vc / (invh_vc**2).sum(axis=0)**0.5).T
-        coef_cd = np.zeros((3, D))
-        for c, n_v in enumerate(n_cv):
-            ok_d = abs(h_dv.dot(n_v)) > 1e-10
-            h_mv = h_dv[ok_d]
-            A_jm = np.array([h_mv.dot(n_v),
-                             h_mv.dot(gd.h_cv[c - 1]),
-                             h_mv.dot(gd.h_cv[c - 2])])
-            U_jm, S_m, V_mm = np.linalg.svd(A_jm, full_matrices=False)
-            coef_m = V_mm.T.dot(np.diag(S_m**-1).dot(U_jm.T.dot([1, 0, 0])))
-            coef_cd[c, ok_d] = coef_m
-        coef_d = np.linalg.inv(n_cv)[v].dot(coef_cd) * scale
-        offsets = []
-        coefs = []
-        stencil = np.array(derivatives[n - 1])
-        for d, c in enumerate(coef_d):
-            if abs(c) < 1e-10:
+        M_dc, g_d = gradient_weights(gd.h_cv, v)
+        w = np.array(derivatives[n - 1], float)
+        offsets, coefs = [], []
+        for M_c, g in zip(M_dc, scale * g_d):
+            if abs(g) < 1e-10:
                 continue
-            M_c = M_dc[d]
-            offsets.extend(np.arange(1, n + 1)[:, np.newaxis] * M_c)
-            coefs.extend(c * stencil)
-            offsets.extend(np.arange(-1, -n - 1, -1)[:, np.newaxis] * M_c)
-            coefs.extend(-c * stencil)
+            o, c = _line(M_c, g * w, -1)
+            offsets += o
+            coefs += c
         super().__init__(coefs, offsets, gd, dtype, xp=xp)
         self.description = (
-            'Finite-difference {}-derivative with O(h^{}) error ({} points)'
-            .format('xyz'[v], 2 * n, self.npoints))
+            f'Finite-difference {"xyz"[v]}-derivative with O(h^{2 * n}) '
+            f'error ({self.npoints} points)')
+
+
+# ----------------------------------------------------------- Mehrstellen
+#
+# The compact O(h^4) scheme A phi = B f for lap phi = f on an orthorhombic
+# grid (Collatz).  With q_v = 1 / (12 h_v^2) and Q = q_x + q_y + q_z:
+#   A: -16 Q at the centre, 10 q_v - 2 Q on the two faces along v,
+#      q_u + q_w on the four edges of the (u, w) plane;
+#   B: 1/2 at the centre, 1/12 on the six faces.
+# For a cube A = (1 / 6h^2) [-24 (centre), 2 (faces), 1 (edges)].
+# LaplaceA(gd, scale) is -scale * A (the Poisson solver passes
+# scale = 1 / 4 pi for -lap / 4 pi, gpaw/poisson.py:371-378).
+
+_FACES = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1),
+          (0, 0, 1)]
+_EDGES = [(0, -1, -1), (0, -1, 1), (0, 1, -1), (0, 1, 1),
+          (-1, 0, -1), (-1, 0, 1), (1, 0, -1), (1, 0, 1),
+          (-1, -1, 0), (-1, 1, 0), (1, -1, 0), (1, 1, 0)]
 
 
 class LaplaceA(FDOperator):
     def __init__(self, gd, scale, dtype=float, xp=np):
-        assert gd.orthogonal
-        c = np.divide(-1 / 12, gd.h_cv.diagonal()**2) * scale
-        c0 = c[1] + c[2]
-        c1 = c[0] + c[2]
-        c2 = c[1] + c[0]
-        a = -16.0 * np.sum(c)
-        b = 10.0 * c + 0.125 * a
-        super().__init__([a,
-                          b[0], b[0],
-                          b[1], b[1],
-                          b[2], b[2],
-                          c0, c0, c0, c0,
-                          c1, c1, c1, c1,
-                          c2, c2, c2, c2],
-                         [(0, 0, 0),
-                          (-1, 0, 0), (1, 0, 0),
-                          (0, -1, 0), (0, 1, 0),
-                          (0, 0, -1), (0, 0, 1),
-                          (0, -1, -1), (0, -1, 1), (0, 1, -1), (0, 1, 1),
-                          (-1, 0, -1), (-1, 0, 1), (1, 0, -1), (1, 0, 1),
-                          (-1, -1, 0), (-1, 1, 0), (1, -1, 0), (1, 1, 0)],
-                         gd, dtype,
-                         'O(h^4) Mehrstellen Laplacian (A)',
-                         xp=xp)
+        """Left-hand operator of the Mehrstellen pair, -scale * A; point
+        order: centre, faces (x, y, z; - then +), edges of the yz, xz and
+        xy planes."""
+        if not gd.orthogonal:
+            raise AssertionError('Mehrstellen stencil: orthogonal cells '
+                                 'only')
+        q = 1.0 / (12.0 * gd.h_cv.diagonal()**2)
+        Q = q.sum()
+        face = [10.0 * q[v] - 2.0 * Q for v in range(3)]
+        plane = [q[1] + q[2], q[0] + q[2], q[0] + q[1]]
+        A = [-16.0 * Q]
+        A += [face[v] for v in range(3) for _ in range(2)]
+        A += [plane[p] for p in range(3) for _ in range(4)]
+        super().__init__([-scale * c for c in A],
+                         [(0, 0, 0)] + _FACES + _EDGES, gd, dtype,
+                         'O(h^4) Mehrstellen Laplacian (A)', xp=xp)
 
 
 class LaplaceB(FDOperator):
     def __init__(self, gd, dtype=float, xp=np):
-        a = 0.5
-        b = 1.0 / 12.0
-        super().__init__([a, b, b, b, b, b, b],
-                         [(0, 0, 0),
-                          (-1, 0, 0), (1, 0, 0),
-                          (0, -1, 0), (0, 1, 0),
-                          (0, 0, -1), (0, 0, 1)],
-                         gd, dtype,
-                         'O(h^4) Mehrstellen Laplacian (B)',
-                         xp=xp)
+        """Right-hand operator of the Mehrstellen pair: 1/2 at the centre,
+        1/12 on the six faces."""
+        super().__init__([0.5] + [1.0 / 12.0] * 6, [(0, 0, 0)] + _FACES, gd,
+                         dtype, 'O(h^4) Mehrstellen Laplacian (B)', xp=xp)
 
 
 __all__ = ['FDOperator', 'Laplace', 'GUCLaplace', 'Gradient', 'LaplaceA',
-           'LaplaceB', 'find_neighbors', 'minkowski_reduce', 'laplace',
+           'LaplaceB', 'find_neighbors', 'minkowski_reduce', 'voronoi_steps',
+           'laplacian_directions', 'gradient_weights', 'laplace',
            'derivatives', 'pi']

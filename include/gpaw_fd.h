@@ -378,6 +378,16 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 
+/* The full-row tiles the tuner times for a stencil radius, data type
+ * (0 f64, 1 c128, 2 f32, 3 c64) and grid shape, besides its heuristic one:
+ * up to max_codes of them are written to codes in the "xr_tile" encoding
+ * (tz | nseg << 12 | py << 24 | xchunk << 32 | dma << 48 | order << 52 |
+ * wide << 56); with_dma adds the LDS-DMA tiles.  Returns how many there
+ * are (-1 on bad arguments).  No reference counterpart: it lets tests run
+ * every tile the tuner may pick. */
+int gfd_row_tile_candidates(int radius, int dtype, const long n[3],
+                            int with_dma, long* codes, int max_codes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,27 +36,75 @@ def tile_code(n2, nseg, py, xchunk, A):
     return n2 | (nseg << 12) | (py << 24) | (xchunk << 32) | (A << 48)
 
 
-def tiles_for(n, R, rng, count=3):
-    cands = [(s, p, A) for s in range(1, 9) for p in (1, 2, 3, 4)
-             for A in (1, 2) if tile_ok(n, R, s, p, A)]
-    pick = rng.choice(len(cands), size=min(count, len(cands)),
-                      replace=False)
-    return [cands[i] for i in pick]
-
-
 def ref_apply(op, a):
     return oracle.fd_apply(op.coef_p, op.offset_pc, op.gd.n_c,
                            op.gd.neighbor_cd, a)
 
 
+# the (radius, rows per thread, halo chunks per thread) instantiations of
+# k_xdma (rows_dma.h); halo chunks = ceil(2 R / row segments)
+XD_CASES = [(R, PY, H) for R in (1, 2, 3, 4) for PY in (1, 2, 3, 4)
+            for H in (1, 2)]
+BCS = [(True, True, True), (False, False, False), (True, False, True),
+       (False, True, False)]
+
+
+def case_grid(R, PY, H):
+    """A grid shape and row-segment count that the (R, PY, H) instance
+    takes: 2 tiles along y, 64 or 128 points along z."""
+    for n2 in (64, 128):
+        for nseg in range(1, 17):
+            if -(-2 * R // nseg) != H:
+                continue
+            n = (R + 7, 2 * nseg * PY, n2)
+            if tile_ok(n, R, nseg, PY, 1):
+                return n, nseg
+    raise AssertionError(f'no grid for {(R, PY, H)}')
+
+
+@pytest.mark.parametrize('R,PY,H', XD_CASES)
+def test_xdma_every_instance(R, PY, H):
+    """Each k_xdma instantiation on a grid and tile it takes, 1 and 2
+    planes in flight, whole and 5-plane x chunks, the boundary kinds in
+    turn (one in four cases all-zero)."""
+    lib = _lib.load()
+    i = XD_CASES.index((R, PY, H))
+    rng = np.random.default_rng(700 + i)
+    n, nseg = case_grid(R, PY, H)
+    pbc = BCS[i % len(BCS)]
+    N = tuple(x if p else x + 1 for x, p in zip(n, pbc))
+    gd = GridDescriptor(N, (3.0, 2.0, 4.0), pbc)
+    assert tuple(gd.n_c) == n
+    op = Laplace(gd, -0.5, R)
+    a = rand(rng, (3,) + n)
+    ref = ref_apply(op, a)
+    for A in (1, 2):
+        assert tile_ok(n, R, nseg, PY, A)
+        for xc in (n[0], 5):
+            say('xdma', (R, PY, H), n, pbc, nseg, A, xc)
+            try:
+                _lib.check(lib.gfd_set_tuning(
+                    b'xr_tile', tile_code(n[2], nseg, PY, xc, A)), 'tuning')
+                with kernel_path('row'):
+                    d = dev(a)
+                    out = d.new_empty(d.shape)
+                    op.apply(d, out)
+                    got = host(out)
+            finally:
+                lib.gfd_set_tuning(b'xr_tile', 0)
+            assert relerr(got, ref) < TOL, (n, pbc, nseg, PY, A, xc)
+
+
 CASES = [((16, 24, 32), (True, True, True)),
-         ((20, 12, 65), (False, False, False)),
+         ((21, 13, 64), (False, False, False)),
          ((13, 8, 96), (True, False, True)),
          ((9, 16, 49), (False, True, False))]
 
 
 @pytest.mark.parametrize('nn', [1, 2, 3, 4])
 def test_xdma_laplace(nn):
+    """Every tile that fits each grid (rows per thread 1-4, 1-2 planes in
+    flight), whole-grid and 5-plane x chunks."""
     lib = _lib.load()
     rng = np.random.default_rng(70 + nn)
     for N, pbc in CASES:
@@ -67,8 +115,11 @@ def test_xdma_laplace(nn):
         op = Laplace(gd, -0.5, nn)
         a = rand(rng, (3,) + n)
         ref = ref_apply(op, a)
-        for nseg, py, A in tiles_for(n, nn, rng):
-            for xc in (n[0], 5, 16):
+        tiles = [(s, p, A) for s in range(1, 9) for p in (1, 2, 3, 4)
+                 for A in (1, 2) if tile_ok(n, nn, s, p, A)]
+        assert tiles, (n, nn)
+        for nseg, py, A in tiles:
+            for xc in (n[0], 5):
                 xc = min(xc, n[0])
                 say('xdma', n, pbc, nn, nseg, py, A, xc)
                 try:

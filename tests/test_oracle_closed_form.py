@@ -262,3 +262,71 @@ def test_vcycle_converges_to_spectral_solution(nn):
     ref = spectral_solution(s.operators[0], rho)
     ref -= ref.mean()
     assert np.abs(phi - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_mehrstellen_pair_exact_to_degree_5():
+    """LaplaceA p = LaplaceB lap(p) (scale -1) at interior points for every
+    monomial of degree <= 5 on an anisotropic grid: the defining property
+    of the O(h^4) compact scheme.  Degree 6 must fail (the check is
+    sensitive)."""
+    from gpaw_amd.fd_operators import LaplaceA, LaplaceB
+    gd = GridDescriptor((14, 16, 12), (14 * 0.31, 16 * 0.27, 12 * 0.35),
+                        False)
+    A = LaplaceA(gd, -1.0)
+    B = LaplaceB(gd)
+    x, y, z = gd.get_grid_point_coordinates()
+    inner = (slice(2, -2),) * 3
+
+    def d2(e, v):
+        return e * (e - 1) * v**(e - 2) if e >= 2 else 0 * v
+
+    def mismatch(a, b, c):
+        p = x**a * y**b * z**c
+        lap = (d2(a, x) * y**b * z**c + x**a * d2(b, y) * z**c +
+               x**a * y**b * d2(c, z))
+        Ap = oracle.fd_apply(A.coef_p, A.offset_pc, gd.n_c, gd.neighbor_cd,
+                             p)[inner]
+        Bl = oracle.fd_apply(B.coef_p, B.offset_pc, gd.n_c, gd.neighbor_cd,
+                             lap)[inner]
+        # against the size of the terms the stencils sum
+        scale = (np.abs(A.coef_p).sum() * np.abs(p).max() +
+                 np.abs(B.coef_p).sum() * np.abs(lap).max())
+        return np.abs(Ap - Bl).max() / scale
+
+    for a in range(6):
+        for b in range(6 - a):
+            for c in range(6 - a - b):
+                assert mismatch(a, b, c) < 1e-12, (a, b, c)
+    assert mismatch(6, 0, 0) > 1e-9
+    assert mismatch(2, 4, 0) > 1e-9
+
+
+@pytest.mark.parametrize('cell,pairs', [
+    ([1.1, 1.02, 1.03], 3),                        # simple cubic: 6 faces
+    ([(0, 1, 1), (1, 0, 1), (1, 1, 0)], 6),        # fcc: 12 faces
+    ([(-1, 1, 1), (1, -1, 1), (1, 1, -1)], 7),     # bcc: 14 faces
+    ([(1, 0, 0), (0.5, 3**0.5 / 2, 0), (0, 0, 1.1)], 4)])  # hexagonal: 8
+def test_voronoi_face_counts(cell, pairs):
+    """Number of faces of the Voronoi cell (the Wigner-Seitz cell) of the
+    textbook lattices, and every neighbour step at most as long as the
+    next."""
+    from gpaw_amd.fd_operators import find_neighbors
+    h = np.asarray(cell, float)
+    h = np.diag(h) if h.ndim == 1 else h
+    M = find_neighbors(h)
+    assert len(M) == pairs
+    L = ((M @ h)**2).sum(1)
+    assert (np.diff(L) >= -1e-12).all()
+
+
+def test_laplacian_tensor_identity():
+    """sum_d a_d h_d h_d^T = 1 for the directions and weights of every
+    test cell."""
+    from gpaw_amd.fd_operators import laplacian_directions
+    for name, D, cell in CELLS:
+        gd = GridDescriptor((8, 8, 8), cell)
+        M, a = laplacian_directions(gd.h_cv)
+        h = M @ gd.h_cv
+        T = np.einsum('d,dv,dw->vw', a, h, h)
+        assert len(a) == D, name
+        assert np.abs(T - np.eye(3)).max() < 1e-12, name

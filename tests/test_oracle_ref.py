@@ -1,7 +1,8 @@
 """The oracle against the reference extension built from the reference
 sources (oracle/_ref), on randomised stencils, boundaries and transfers.
-Skipped where the reference build is absent (it never leaves the build
-container's tree of sources)."""
+Skipped where that build is absent.  The built library travels with the
+tree (it is what bench.py times as the CPU baseline), but the reference
+sources do not: it is only rebuilt where they are present."""
 import glob
 import os
 import sys
