@@ -20,6 +20,7 @@
 #include "bands.h"
 #include "handles.h"
 #include "coarse.h"
+#include "gs.h"
 
 namespace gfd {
 
@@ -330,12 +331,7 @@ static int check_relax(const gfd_operator* op, int method)
         set_error("relax: real data only");
         return EINVAL_;
     }
-    if (method == 1) {
-        set_error("relax: Gauss-Seidel (method 1) is not implemented on the "
-                  "GPU; use Jacobi (method 2)");
-        return ENOTSUP_;
-    }
-    if (method != 2) {
+    if (method != 1 && method != 2) {
         set_error("relax: unknown method %d", method);
         return EINVAL_;
     }
@@ -344,6 +340,28 @@ static int check_relax(const gfd_operator* op, int method)
         return EINVAL_;
     }
     return OK;
+}
+
+// nsweeps Gauss-Seidel sweeps on f in place (c/bmgs/relax.cpp:12-40);
+// snap: a work grid of the operator's size.
+static int op_gauss_seidel(gfd_operator* op, double* f, double* snap,
+                           const double* src, int nsweeps, hipStream_t s)
+{
+    if (op->gs_a0 == 0)
+        GFD_TRY(gs_order(op->off.data(), op->P, &op->gs_a0, &op->gs_a1));
+    GsArgs a{};
+    for (int c = 0; c < 3; c++) {
+        a.n[c] = op->n[c];
+        a.periodic[c] = op->periodic[c];
+    }
+    a.a0 = op->gs_a0;
+    a.a1 = op->gs_a1;
+    a.nw = 0;
+    a.P = op->P;
+    a.coef = op->d_coef;
+    a.off = op->d_off;
+    a.inv_c0 = 1.0 / op->coef[0];
+    return gs_relax(a, f, snap, src, nsweeps, s);
 }
 
 extern "C" int gfd_operator_relax(gfd_operator* op, int method, double* f,
@@ -358,6 +376,10 @@ extern "C" int gfd_operator_relax(gfd_operator* op, int method, double* f,
     if (nrelax == 0) return OK;
     hipStream_t s = (hipStream_t)stream;
     const long ng = op->ng();
+    if (method == 1) {
+        GFD_TRY(op->scratch.ensure(sizeof(double) * ng, s));
+        return op_gauss_seidel(op, f, (double*)op->scratch.p, src, nrelax, s);
+    }
     GFD_TRY(op->scratch.ensure(2 * sizeof(double) * ng, s));
     double* t1 = (double*)op->scratch.p;
     double* t2 = t1 + ng;
@@ -734,6 +756,7 @@ static long g_coarse_max_pts = 4096;
 
 struct gfd_poisson {
     int L = 0;  // number of coarsenings
+    int method = 2;  // FP64 smoother: 2 weighted Jacobi, 1 Gauss-Seidel
     std::vector<gfd_operator*> ops;
     std::vector<gfd_transformer*> rst, itp;
     std::vector<int> pre, post;
@@ -829,6 +852,7 @@ extern "C" int gfd_poisson_new(int nlevels, gfd_operator* const* ops,
     p->post.assign(postsmooths, postsmooths + nlevels + 1);
     p->w.assign(weights, weights + nlevels + 1);
     p->dv = dv;
+    p->method = relax_method;
     int st = alloc_levels(p, p->lv, 0, nlevels);
     // every norm of the drivers is over level 0 (one grid); transfers of
     // either precision use at most the FP64 scratch size
@@ -873,10 +897,16 @@ extern "C" void gfd_poisson_free(gfd_poisson* p)
     delete p;
 }
 
-// nrelax Jacobi sweeps of any precision ping-ponging between *cur / *alt
+// nrelax smoothing sweeps: Jacobi of any precision ping-ponging between
+// *cur / *alt, or (method 1, FP64) Gauss-Seidel in place on *cur with *alt
+// as its start-of-sweep copy
 static int relax_pp_t(gfd_operator* op, int dtype, void** cur, void** alt,
-                      const void* src, int nrelax, double w, hipStream_t s)
+                      const void* src, int nrelax, double w, hipStream_t s,
+                      int method = 2)
 {
+    if (method == 1 && dtype == 0)
+        return op_gauss_seidel(op, (double*)*cur, (double*)*alt,
+                               (const double*)src, nrelax, s);
     EpiArgs ea{};
     ea.aux = src;
     ea.c0 = op->coef[0];
@@ -926,7 +956,10 @@ static int plan_coarse(gfd_poisson* p, const Schedule& sc, int dtype,
     const size_t es = dtype == 2 ? 4 : 8;
     int lc = -1;
     size_t bytes = 0;
-    for (int l0 = 1; g_coarse_lds && l0 <= sc.coarsest; l0++) {
+    // the LDS-resident solve smooths with Jacobi: not for the FP64
+    // Gauss-Seidel schedule
+    const bool lds_ok = g_coarse_lds && !(dtype == 0 && p->method == 1);
+    for (int l0 = 1; lds_ok && l0 <= sc.coarsest; l0++) {
         if (sc.coarsest - l0 + 1 > CMAXLEV) continue;
         if (p->ops[l0]->ng() > g_coarse_max_pts) continue;
         size_t b = 0;
@@ -1016,7 +1049,7 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
     const size_t es = dt == 2 ? 4 : 8;
     if (l < sc.coarsest) {
         GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_pre(sc, l),
-                           sched_w(sc, l), s));
+                           sched_w(sc, l), s, p->method));
         EpiArgs e{};
         e.aux = rho;
         e.c0 = centre_coef(op);
@@ -1044,7 +1077,7 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
         std::swap(*cur, *alt);
     }
     GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_post(sc, l),
-                       sched_w(sc, l), s));
+                       sched_w(sc, l), s, p->method));
     return OK;
 }
 
@@ -1611,6 +1644,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
     }
     if (std::strcmp(key, "xfer_passes") == 0) {
         xfer_force_passes(value != 0);
+        return OK;
+    }
+    if (std::strcmp(key, "xfer_march") == 0) {
+        xfer_set_march(value != 0);
         return OK;
     }
     if (std::strcmp(key, "xr_tile") == 0) {

@@ -69,6 +69,7 @@ struct gfd_operator {
     int* d_off = nullptr;
     DevBuf scratch;         // relax ping-pong / in-place copies
     DevBuf partial;         // residual-norm partial sums
+    int gs_a0 = 0, gs_a1 = 0;  // Gauss-Seidel wavefront order (0: unset)
     DevBuf scalar;          // one device double
     long ng() const { return (long)n[0] * n[1] * n[2]; }
     int dtype() const { return real ? 0 : 1; }

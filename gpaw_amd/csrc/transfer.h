@@ -42,6 +42,15 @@ struct XferPlan {
 
 // K = 2 transfers run as one fused kernel unless forced to the 3 passes.
 void xfer_force_passes(bool on);
+// Use the x-marching fused kernels (every K; default on).  Off: K = 2 by
+// the one-point-per-thread fused kernels, K > 2 by three 1-D passes.
+void xfer_set_march(bool on);
+
+// The x-marching fused transfer (xfer_march.hip); ENOTSUP_ for a shape
+// its tiles cannot take.
+int launch_transfer_march(const XferPlan& p, int dtype, const BC& bc,
+                          const void* in, void* out, long nin,
+                          const XEpiArgs& ea, hipStream_t stream);
 
 // Scratch (in elements of the data type) needed for `nin` grids.
 long xfer_scratch_elems(const XferPlan& p, long nin);

@@ -515,8 +515,10 @@ static int run_transfer_k(const XferPlan& p, const BC& bc, const void* in,
 }
 
 static bool g_xfer_force_passes = false;
+static bool g_xfer_march = true;
 
 void xfer_force_passes(bool on) { g_xfer_force_passes = on; }
+void xfer_set_march(bool on) { g_xfer_march = on; }
 
 int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
                     const void* in, void* out, long nin, void* scratch,
@@ -525,6 +527,13 @@ int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
     if (ea.in_wide && (p.K != 2 || (dtype != 2 && dtype != 3))) {
         set_error("transfer: FP64 input only for FP32 data and K = 2");
         return EINVAL_;
+    }
+    if (g_xfer_march && !g_xfer_force_passes) {
+        // the x-marching fused kernel (xfer_march.hip), any K
+        const int st =
+            launch_transfer_march(p, dtype, bc, in, out, nin, ea, stream);
+        if (st != ENOTSUP_) return st;
+        set_error("");
     }
     if (dtype == 3 && p.K != 2) {
         set_error("transfer: complex64 data only for K = 2");

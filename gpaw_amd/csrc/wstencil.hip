@@ -15,6 +15,7 @@
 #include <cstring>
 #include <vector>
 
+#include "gs.h"
 #include "handles.h"
 #include "stencil_dev.h"
 
@@ -188,6 +189,8 @@ struct gfd_woperator {
     const double** d_wts = nullptr;
     int R = 0;
     long* d_foff = nullptr;
+    std::vector<int> h_off;   // point offsets (o0, o1, o2), host copy
+    int gs_a0 = 0, gs_a1 = 0; // Gauss-Seidel wavefront order (0: unset)
     DevBuf scratch;
     DevBuf partial;
     long ng() const { return (long)n[0] * n[1] * n[2]; }
@@ -316,6 +319,7 @@ extern "C" int gfd_woperator_new(int nweights, const int* ncoefs,
         for (int c = 0; c < 3; c++) R = std::max(R, std::abs(o[c]));
     }
     op->R = R;
+    op->h_off = off;
     if (hipMalloc(&op->d_foff, sizeof(long) * total) != hipSuccess ||
         hipMemcpy(op->d_foff, fo.data(), sizeof(long) * total,
                   hipMemcpyHostToDevice) != hipSuccess) {
@@ -385,16 +389,35 @@ extern "C" int gfd_woperator_relax(gfd_woperator* op, int method, double* f,
         set_error("gfd_woperator_relax: real data only");
         return EINVAL_;
     }
-    if (method != 2) {
-        set_error("gfd_woperator_relax: only weighted Jacobi (method 2) runs "
-                  "on the GPU");
-        return ENOTSUP_;
+    if (method != 1 && method != 2) {
+        set_error("gfd_woperator_relax: unknown method %d", method);
+        return EINVAL_;
     }
     if (nrelax == 0) return OK;
     hipStream_t s = (hipStream_t)stream;
     const long ng = op->ng();
     GFD_TRY(op->scratch.ensure(sizeof(double) * ng, s));
     double* tmp = (double*)op->scratch.p;
+    if (method == 1) {
+        // lexicographic Gauss-Seidel (c/bmgs/wrelax.cpp:26-58), tmp holds
+        // the start-of-sweep values
+        if (op->gs_a0 == 0)
+            GFD_TRY(gs_order(op->h_off.data(), (int)(op->h_off.size() / 3),
+                             &op->gs_a0, &op->gs_a1));
+        GsArgs g{};
+        for (int c = 0; c < 3; c++) {
+            g.n[c] = op->n[c];
+            g.periodic[c] = op->periodic[c];
+        }
+        g.a0 = op->gs_a0;
+        g.a1 = op->gs_a1;
+        g.nw = op->nw;
+        g.start = op->d_start;
+        g.coef = op->d_coef;
+        g.off = op->d_off;
+        g.wts = op->d_wts;
+        return gs_relax(g, f, tmp, src, nrelax, s);
+    }
     const WArgs a = op->args(nullptr);
     const unsigned g = wgrid(ng);
     // sweeps alternate f -> tmp -> f; an odd count ends with a copy back

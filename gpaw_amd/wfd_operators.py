@@ -134,8 +134,8 @@ class WeightedFDOperator:
         st.download(1)
 
     def relax(self, relax_method, f_g, s_g, n, w=None):
-        """n weighted Jacobi sweeps (relax_method 2) for A f = s; method 1
-        (Gauss-Seidel) is not run on the GPU."""
+        """n sweeps for A f = s: relax_method 2 weighted Jacobi, 1
+        lexicographic Gauss-Seidel (w unused), as c/bmgs/wrelax.cpp."""
         if self.dtype != float:
             raise TypeError('relax: real operators only')
         if tuple(f_g.shape) != self.shape or tuple(s_g.shape) != self.shape:

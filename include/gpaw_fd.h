@@ -374,7 +374,9 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * "coarse_max_points" (default and 0: 4096) caps the points of its
  * first level;
  * "xfer_passes" (0/1) runs K = 2 transfers as three 1-D passes instead of
- * the fused kernel.
+ * the fused kernel; "xfer_march" (0/1, default 1) runs every transfer
+ * through the x-marching fused kernel (off: one point per thread for
+ * K = 2, three 1-D passes for K > 2).
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

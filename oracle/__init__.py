@@ -279,9 +279,9 @@ class OracleBackend:
         fd_apply(op.coef_p, op.offset_pc, op.n_c, op.neighbor_cd, a,
                  phases, out=out, nthreads=self.nthreads)
 
-    def relax(self, op, f, src, n, w):
-        fd_relax(2, op.coef_p, op.offset_pc, op.n_c, op.neighbor_cd, f, src,
-                 n, w)
+    def relax(self, op, f, src, n, w, method=2):
+        fd_relax(method, op.coef_p, op.offset_pc, op.n_c, op.neighbor_cd, f,
+                 src, n, w)
 
     def transform(self, t, a, out, phases=None):
         transform(t.interpolate, t.k, t.size_in, t.size_out, t.pad_cd,
@@ -329,11 +329,13 @@ class PoissonVCycle:
 
     ops[l] (StencilSpec), restrictors[l] / interpolators[l] (TransferSpec)
     for l = 0..L; smoothing counts and Jacobi weights as the reference's
-    set_grid_descriptor / _init produce them.
+    set_grid_descriptor / _init produce them; relax_method 2 (Jacobi) or
+    1 (Gauss-Seidel), passed to every relax call as the reference's
+    iterate2 does.
     """
 
     def __init__(self, ops, restrictors, interpolators, presmooths,
-                 postsmooths, weights, dv, backend=None):
+                 postsmooths, weights, dv, backend=None, relax_method=2):
         self.ops = ops
         self.restrictors = restrictors
         self.interpolators = interpolators
@@ -343,16 +345,24 @@ class PoissonVCycle:
         self.levels = len(ops) - 1
         self.dv = dv
         self.be = backend or OracleBackend()
+        self.method = relax_method
         self.phis = [None] + [np.empty(tuple(o.n_c)) for o in ops[1:]]
         self.rhos = [np.empty(tuple(o.n_c)) for o in ops]
         self.residuals = [np.empty(tuple(o.n_c)) for o in ops]
+
+    def _relax(self, op, level, n):
+        if self.method == 2:
+            self.be.relax(op, self.phis[level], self.rhos[level], n,
+                          self.weights[level])
+        else:
+            self.be.relax(op, self.phis[level], self.rhos[level], n,
+                          self.weights[level], self.method)
 
     def iterate2(self, level=0):
         residual = self.residuals[level]
         op = self.ops[level]
         if level < self.levels:
-            self.be.relax(op, self.phis[level], self.rhos[level],
-                          self.presmooths[level], self.weights[level])
+            self._relax(op, level, self.presmooths[level])
             self.be.apply(op, self.phis[level], residual)
             residual -= self.rhos[level]
             self.be.transform(self.restrictors[level], residual,
@@ -362,8 +372,7 @@ class PoissonVCycle:
             self.be.transform(self.interpolators[level],
                               self.phis[level + 1], residual)
             self.phis[level] -= residual
-        self.be.relax(op, self.phis[level], self.rhos[level],
-                      self.postsmooths[level], self.weights[level])
+        self._relax(op, level, self.postsmooths[level])
         if level == 0:
             self.be.apply(op, self.phis[level], residual)
             residual -= self.rhos[level]

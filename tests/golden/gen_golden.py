@@ -79,8 +79,9 @@ class RefBackend:
             o.apply(np.ascontiguousarray(a), out,
                     np.ones((3, 2), complex) if phases is None else phases)
 
-    def relax(self, op, f, src, n, w):
-        ref_operator(op, True).relax(2, f, np.ascontiguousarray(src), n, w)
+    def relax(self, op, f, src, n, w, method=2):
+        ref_operator(op, True).relax(method, f, np.ascontiguousarray(src), n,
+                                     w)
 
     def transform(self, t, a, out, phases=None):
         real = a.dtype == np.float64

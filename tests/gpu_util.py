@@ -37,3 +37,15 @@ def kernel_path(name='general'):
 
 def general_path(on=True):
     return kernel_path('general' if on else 'auto')
+
+
+@contextlib.contextmanager
+def xfer_path(name='march'):
+    """Route transfers through the x-marching kernel ('march', default) or
+    the one-point-per-thread / 1-D pass kernels ('legacy') while active."""
+    lib = _lib.load()
+    lib.gfd_set_tuning(b'xfer_march', 1 if name == 'march' else 0)
+    try:
+        yield
+    finally:
+        lib.gfd_set_tuning(b'xfer_march', 1)

@@ -22,7 +22,8 @@ def vcycle(solver, backend=None):
                                 [tspec(t) for t in solver.restrictors],
                                 [tspec(t) for t in solver.interpolators],
                                 solver.presmooths, solver.postsmooths,
-                                solver.weights, solver.gd.dv, backend)
+                                solver.weights, solver.gd.dv, backend,
+                                getattr(solver, 'relax_method', 2))
 
 
 def relerr(x, ref):

@@ -6,7 +6,7 @@ import oracle
 from gpaw_amd.fd_operators import Laplace
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpaw_amd.preconditioner import Preconditioner
-from gpu_util import dev, host, say
+from gpu_util import dev, host, say, xfer_path
 from helpers import bloch_phases, case, rand, relerr, spec, tspec
 
 pytestmark = pytest.mark.gpu
@@ -52,9 +52,10 @@ def test_blocks_and_host_arrays():
     assert relerr(one, ref[2]) < TOL
 
 
+@pytest.mark.parametrize('path', ['march', 'legacy'])
 @pytest.mark.parametrize('dtype,pbc', [(complex, True), (float, True),
                                        (float, False)])
-def test_fp32_chain(dtype, pbc):
+def test_fp32_chain(dtype, pbc, path):
     """FP32 chain against the FP64 oracle: float32 rounding over ~10
     passes, so a relative max-norm error of a few 1e-7 (bar 2e-6)."""
     rng = np.random.default_rng(6)
@@ -68,8 +69,9 @@ def test_fp32_chain(dtype, pbc):
                             tspec(P.restrictor_object1),
                             tspec(P.interpolator_object1),
                             tspec(P.interpolator_object2), res, P.step, ph)
-    say('precond fp32', dtype.__name__, pbc)
-    out = host(P(dev(res), K(ph)))
+    say('precond fp32', dtype.__name__, pbc, path)
+    with xfer_path(path):
+        out = host(P(dev(res), K(ph)))
     assert out.dtype == res.dtype
     err = relerr(out, ref)
     assert err < 2e-6

@@ -50,12 +50,87 @@ def test_relax_sweep_counts():
             assert relerr(host(f), ref) < TOL
 
 
-def test_gauss_seidel_not_on_gpu():
-    gd = GridDescriptor((8, 8, 8))
-    op = Laplace(gd, 1.0, 1)
-    f = dev(np.zeros((8, 8, 8)))
-    with pytest.raises(NotImplementedError, match='Gauss-Seidel'):
-        op.relax(1, f, f.clone(), 1)
+GS_CASES = [
+    # (kind, N, pbc): small grids run in one workgroup, the 72 x 64 one
+    # with one launch per wavefront step
+    ('nn1', (12, 10, 14), True),
+    ('nn3', (14, 12, 10), (True, False, True)),
+    ('nn3', (72, 64, 16), True),
+    ('mehrstellen', (12, 14, 10), False),
+    ('mehrstellen', (72, 66, 8), (False, True, True)),
+    ('hexagonal nn2', (12, 12, 10), True),
+]
+
+
+def _gs_operator(kind, gd):
+    from gpaw_amd.fd_operators import GUCLaplace, LaplaceA
+    if kind == 'mehrstellen':
+        return LaplaceA(gd, 0.25 / np.pi)
+    if kind.startswith('hexagonal'):
+        return GUCLaplace(gd, -0.25 / np.pi, 2)
+    return Laplace(gd, -0.25 / np.pi, int(kind[2]))
+
+
+@pytest.mark.parametrize('kind,N,pbc', GS_CASES)
+def test_gauss_seidel_bit_exact(kind, N, pbc):
+    """Lexicographic Gauss-Seidel (relax method 1) in wavefront order is the
+    reference's sweep bit for bit: same neighbour values, same sums."""
+    rng = np.random.default_rng(40 + GS_CASES.index((kind, N, pbc)))
+    cell = (np.array([(1, 0, 0), (0.5, 3**0.5 / 2, 0), (0, 0, 1.1)]) *
+            np.array(N)[:, None] * 0.3 if kind.startswith('hexagonal')
+            else np.array(N) * 0.3)
+    gd = GridDescriptor(N, cell, pbc)
+    op = _gs_operator(kind, gd)
+    f0 = rand(rng, tuple(gd.n_c))
+    src = rand(rng, tuple(gd.n_c))
+    for n in (1, 3):
+        ref = f0.copy()
+        oracle.fd_relax(1, op.coef_p, op.offset_pc, gd.n_c, gd.neighbor_cd,
+                        ref, src, n, 0.7)
+        f = dev(f0)
+        say('gauss-seidel', kind, N, pbc, n)
+        op.relax(1, f, dev(src), n, 0.7)
+        got = host(f)
+        assert (got == ref).all(), relerr(got, ref)
+
+
+@pytest.mark.parametrize('nn,pbc', [(3, False), (1, True), ('M', False),
+                                    ('M', True)])
+def test_vcycles_gauss_seidel_and_mehrstellen(nn, pbc):
+    """FDPoissonSolver(relax='GS') and nn='M' (B rho, then V-cycles on
+    LaplaceA) against the oracle V-cycle with the same hierarchy, four
+    cycles from phi = 0."""
+    relax = 'J' if nn == 'M' and pbc else 'GS'
+    rng = np.random.default_rng(17)
+    gd = GridDescriptor((16, 16, 24), (4.0, 4.0, 6.0), pbc)
+    rho = rand(rng, tuple(gd.n_c))
+    if gd.pbc_c.all():
+        rho -= rho.mean()
+    s = FDPoissonSolver(nn=nn, relax=relax, eps=1e-30)
+    s.set_grid_descriptor(gd)
+    V = vcycle(s)
+    rhs = rho
+    if nn == 'M':
+        B = s.B
+        rhs = oracle.fd_apply(B.coef_p, B.offset_pc, gd.n_c, gd.neighbor_cd,
+                              rho)
+    V.phis[0] = np.zeros(tuple(gd.n_c))
+    V.rhos[0][:] = rhs
+    phi = dev(np.zeros(tuple(gd.n_c)))
+    drhs = dev(rhs)
+    for i in range(4):
+        e_ref = V.iterate2()
+        say('V-cycle', nn, relax, pbc, i)
+        e = s.iterate2(s.step, phi=phi, rho=drhs)
+        assert relerr(host(phi), V.phis[0]) < TOL
+        assert e == pytest.approx(e_ref, rel=1e-10)
+    # the full solve converges and B is applied inside solve_neutral
+    phi2 = dev(np.zeros(tuple(gd.n_c)))
+    s.eps = 1e-20
+    n = s.solve_neutral(phi2, dev(rho))
+    assert 1 < n < 200
+    r = dev(np.zeros(tuple(gd.n_c)))
+    assert s.operators[0].residual(phi2, drhs, r, norm=True) * gd.dv <= 1e-20
 
 
 def test_vcycle_golden_zero16(golden):
@@ -119,3 +194,56 @@ def test_poisson_convergence_error():
     s.set_grid_descriptor(gd)
     with pytest.raises(PoissonConvergenceError):
         s.solve(dev(np.zeros(tuple(gd.n_c))), dev(rho))
+
+
+def _blob(gd, centre, b, Q):
+    r_vG = gd.get_grid_point_coordinates() - np.asarray(centre)[:, None,
+                                                                 None, None]
+    r = np.sqrt(np.maximum((r_vG**2).sum(0), 1e-12))
+    from scipy.special import erf
+    rho = Q * (b / np.pi)**1.5 * np.exp(-b * r * r)
+    return rho, Q * erf(np.sqrt(b) * r) / r
+
+
+@pytest.mark.parametrize('charge_center', [False, True])
+def test_charged_zero_boundary(charge_center):
+    """A charged Gaussian blob in a box with zero boundaries: the
+    compensating Gaussian carries the charge, the V-cycles solve for the
+    neutral rest, and the result is the free-space potential
+    Q erf(sqrt(b) r) / r of the blob (to the stencil's accuracy)."""
+    gd = GridDescriptor((48, 48, 48), (12.0, 12.0, 12.0), False)
+    centre = (6.6, 5.5, 6.2) if charge_center else (6.0, 6.0, 6.0)
+    rho, exact = _blob(gd, centre, 1.5, 2.0)
+    s = FDPoissonSolver(nn=3, eps=1e-18, use_charge_center=charge_center)
+    s.set_grid_descriptor(gd)
+    phi = dev(np.zeros(tuple(gd.n_c)))
+    say('charged zero-boundary solve, charge centre', charge_center)
+    n = s.solve(phi, dev(rho))
+    assert 1 < n < 200
+    err = relerr(host(phi), exact)
+    say(f'  error against the free-space potential {err:.3g}')
+    assert err < 2e-3
+    # zero_initial_phi restarts from 0 and gives the same potential
+    n2 = s.solve(phi, dev(rho), zero_initial_phi=True)
+    assert relerr(host(phi), exact) < 2e-3 and n2 >= 1
+
+
+def test_charged_periodic_madelung():
+    """Charged periodic cell: the uniform background takes the charge out;
+    with use_charged_periodic_corrections the potential is shifted by
+    charge x Madelung constant of the cell."""
+    from gpaw_amd.poisson import madelung
+    gd = GridDescriptor((32, 32, 32), (8.0, 8.0, 9.0), True)
+    rho, _ = _blob(gd, (4.0, 4.0, 4.5), 2.0, -1.5)
+    q = gd.integrate(rho)
+    out = []
+    for corr in (False, True):
+        s = FDPoissonSolver(nn=3, eps=1e-20,
+                            use_charged_periodic_corrections=corr)
+        s.set_grid_descriptor(gd)
+        phi = dev(np.zeros(tuple(gd.n_c)))
+        say('charged periodic, Madelung correction', corr)
+        s.solve(phi, dev(rho))
+        out.append(host(phi))
+    shift = out[1] - out[0]
+    assert np.abs(shift - q * madelung(gd.cell_cv)).max() < 1e-10

@@ -5,7 +5,7 @@ import pytest
 import oracle
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpaw_amd.transformers import Transformer
-from gpu_util import dev, host, say
+from gpu_util import dev, host, say, xfer_path
 from helpers import bloch_phases, case, case_names, rand, relerr, tspec
 
 pytestmark = pytest.mark.gpu
@@ -44,8 +44,9 @@ def test_golden(golden):
         assert relerr(gpu_transfer(t, c['in'], ph), c['out']) < TOL, name
 
 
+@pytest.mark.parametrize('path', ['march', 'legacy'])
 @pytest.mark.parametrize('nn', [1, 2, 3, 4])
-def test_random(nn):
+def test_random(nn, path):
     rng = np.random.default_rng(100 + nn)
     for N in [(16, 12, 20), (24, 32, 40), (48, 16, 24)]:
         for pbc in [True, False, (True, False, True), (False, False, True)]:
@@ -61,9 +62,11 @@ def test_random(nn):
                     a = rand(rng, (2,) + tuple(gin.n_c), dtype)
                     ph = (bloch_phases(rng, gin.pbc_c)
                           if dtype == complex else None)
-                    say('transfer', nn, N, pbc, interp, dtype.__name__)
-                    assert relerr(gpu_transfer(t, a, ph),
-                                  ref_transfer(t, a, ph)) < TOL
+                    say('transfer', path, nn, N, pbc, interp,
+                        dtype.__name__)
+                    with xfer_path(path):
+                        got = gpu_transfer(t, a, ph)
+                    assert relerr(got, ref_transfer(t, a, ph)) < TOL
 
 
 @pytest.mark.parametrize('pbc', [True, False])

@@ -72,5 +72,12 @@ def test_relax_and_diagonal(pbc):
         assert relerr(host(fd), fr) < TOL
     diag = abs(sum(wt * c[0] for wt, c in zip(w, A.coef_ps))).max()
     assert A.get_diagonal_element() == pytest.approx(diag, rel=1e-14)
-    with pytest.raises(NotImplementedError):
-        A.relax(1, dev(f), dev(src), 1)
+    # lexicographic Gauss-Seidel, bit for bit (c/bmgs/wrelax.cpp:26-58)
+    for nrelax in (1, 2):
+        fr = f.copy()
+        oracle.wfd_relax(1, A.coef_ps, A.offset_pcs, np.array(w), n,
+                         gd.neighbor_cd, fr, src, nrelax, 0.6)
+        say('wfd gauss-seidel', pbc, nrelax)
+        fd = dev(f)
+        A.relax(1, fd, dev(src), nrelax, 0.6)
+        assert (host(fd) == fr).all(), relerr(host(fd), fr)

@@ -104,7 +104,7 @@ def test_poisson_hierarchy():
     assert [o.npoints for o in s.operators] == [19, 7, 7, 7, 7]
     assert s.step == 0.66666666 / s.operators[0].get_diagonal_element()
     d = s.get_description()
-    assert 'Jacobi solver with 5 multi-grid levels' in d
+    assert 'Jacobi multigrid solver, 5 levels' in d
     assert '3 x 3 x 3' in d
     assert s.todict() == {'name': 'fd', 'nn': 3, 'relax': 'J',
                           'eps': 2e-10}
@@ -115,7 +115,7 @@ def test_poisson_warns_on_coarse_grid():
         warnings.simplefilter('always')
         s = FDPoissonSolver()
         s.set_grid_descriptor(GridDescriptor((74, 74, 74)))
-    assert any('Grid unsuitable' in str(x.message) for x in w)
+    assert any('cannot coarsen' in str(x.message) for x in w)
 
 
 def test_errors():
@@ -129,3 +129,41 @@ def test_errors():
         GridDescriptor((8, 8, 8), comm=type('C', (), {'size': 2})())
     with pytest.raises(NotImplementedError):
         Laplace(gd, 1.0, 9)
+
+
+def test_madelung_constants():
+    """Madelung potentials of point-charge lattices in a neutralising
+    background (textbook Wigner-lattice constants, in units of the cubic
+    lattice constant a), independent of the Ewald splitting."""
+    from gpaw_amd.poisson import madelung
+    a = 3.0
+    assert madelung(np.eye(3) * a) * a == pytest.approx(2.8372974794806,
+                                                        rel=1e-12)
+    fcc = np.array([(0, 1, 1), (1, 0, 1), (1, 1, 0)]) * a / 2
+    bcc = np.array([(-1, 1, 1), (1, -1, 1), (1, 1, -1)]) * a / 2
+    assert madelung(fcc) * a == pytest.approx(4.5848622, rel=1e-7)
+    assert madelung(bcc) * a == pytest.approx(3.6392334, rel=1e-7)
+    hexa = np.array([(1, 0, 0), (0.5, 3**0.5 / 2, 0), (0, 0, 1.6)]) * 2.0
+    vals = [madelung(hexa, G=G) for G in (0.6, 1.0, 1.7)]
+    assert max(vals) - min(vals) < 1e-13 * abs(vals[0])
+
+
+def test_gaussian_charge():
+    """The compensating Gaussian holds unit charge and its potential solves
+    lap phi = -4 pi rho (checked with the 19-point stencil inside the
+    box, to its O(h^6) truncation error)."""
+    import oracle
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.poisson import gaussian_charge
+    gd = GridDescriptor((48, 48, 48), (6.0, 6.0, 6.0), False)
+    rho, phi = gaussian_charge(gd)
+    assert gd.integrate(rho) == pytest.approx(1.0, rel=1e-8)
+    lap = Laplace(gd, 1.0, 3)
+    lp = oracle.fd_apply(lap.coef_p, lap.offset_pc, gd.n_c, gd.neighbor_cd,
+                         phi)
+    inner = (slice(8, -8),) * 3
+    err = np.abs(lp[inner] + 4 * np.pi * rho[inner]).max()
+    assert err < 1e-3 * 4 * np.pi * rho.max()
+    # off-centre charge: the width follows the nearest face
+    rho2, _ = gaussian_charge(gd, centre=(2.0, 3.0, 3.5))
+    assert gd.integrate(rho2) == pytest.approx(1.0, rel=1e-6)
