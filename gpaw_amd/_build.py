@@ -33,8 +33,32 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _deps(path: str, seen: set) -> set:
+    """path and every local header it includes, transitively."""
+    import re
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for inc in re.findall(r'^\s*#\s*include\s+"([^"]+)"', f.read(),
+                              re.M):
+            _deps(os.path.normpath(os.path.join(os.path.dirname(path), inc)),
+                  seen)
+    return seen
+
+
+def _stale(src: str, obj: str) -> bool:
+    """Whether obj is older than its source or a header it includes."""
+    if not os.path.exists(obj):
+        return True
+    deps = _deps(os.path.join(CSRC, src), set())
+    return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJ, src.replace('.hip', '.o'))
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, '-x', 'hip', '-c', os.path.join(CSRC, src), '-o', obj]
     cmd += FLAGS
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,11 +67,13 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source (incremental: only those older than their
+    source or a header) and link the library."""
     os.makedirs(OBJ, exist_ok=True)
     workers = min(len(SOURCES), max(1, len(os.sched_getaffinity(0))))
     with ThreadPoolExecutor(workers) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', f'--offload-arch={ARCH}', '-o', tmp]
     cmd += objs
@@ -63,5 +89,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == '__main__':
-    build(verbose=True)
+    build(verbose=True, incremental='--incremental' in sys.argv)
     sys.exit(0)

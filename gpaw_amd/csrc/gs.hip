@@ -145,8 +145,11 @@ int gs_order(const int* off, int npoints, int* a0, int* a1)
         for (int c = 0; c < npoints && ok1; c++) {
             const int* o = off + 3 * c;
             if (o[0] != 0 || (o[1] == 0 && o[2] == 0)) continue;
+            // in C order (0, o1, o2) precedes the centre when o1 < 0, or
+            // o1 = 0 and o2 < 0
+            const bool before = o[1] < 0 || (o[1] == 0 && o[2] < 0);
             const long v = (long)b1 * o[1] + o[2];
-            ok1 = o[1] < 0 ? v < 0 : v > 0;
+            ok1 = before ? v < 0 : v > 0;
         }
         if (!ok1) continue;
         for (int b0 = 1; b0 <= (1 << 20); b0++) {
@@ -167,6 +170,21 @@ int gs_order(const int* off, int npoints, int* a0, int* a1)
     set_error("Gauss-Seidel: no wavefront order for this stencil");
     return ENOTSUP_;
 }
+
+}  // namespace gfd
+
+// Host-only view of the wavefront weights (declared in gpaw_fd.h).
+extern "C" int gfd_gs_order(const int* offsets, int npoints, int* a0,
+                            int* a1)
+{
+    if (!offsets || npoints <= 0 || !a0 || !a1) {
+        gfd::set_error("gfd_gs_order: bad arguments");
+        return gfd::EINVAL_;
+    }
+    return gfd::gs_order(offsets, npoints, a0, a1);
+}
+
+namespace gfd {
 
 int gs_relax(GsArgs a, double* f, double* snap, const double* src,
              int nsweeps, hipStream_t s)

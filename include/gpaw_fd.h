@@ -64,11 +64,22 @@ int gfd_operator_apply(gfd_operator* op, const void* in, void* out, long nin,
 
 /* Replaces Operator.relax_gpu(method, f, src, nrelax[, w]) --
  * c/gpu/operators.cpp:200-221 (CPU: Operator.relax, c/operators.cpp:50-92).
- * method 2 = weighted Jacobi; method 1 (Gauss-Seidel) returns GFD_ENOTSUP,
- * as the reference GPU kernel does not implement it
- * (c/gpu/kernels/relax.cpp:72-80).  Real data only. */
+ * method 2 = weighted Jacobi (c/bmgs/relax.cpp:41-65); method 1 =
+ * lexicographic Gauss-Seidel (c/bmgs/relax.cpp:12-40), bit for bit with the
+ * CPU sweep (the reference GPU kernel has no method 1,
+ * c/gpu/kernels/relax.cpp:72-80).  Real data only. */
 int gfd_operator_relax(gfd_operator* op, int method, double* f,
                        const double* src, int nrelax, double w, void* stream);
+
+/* No counterpart in the reference (host only, no device calls): the step
+ * weights of the Gauss-Seidel sweep of gfd_operator_relax(method 1).  The
+ * sweep updates point (i0, i1, i2) at step a0 i0 + a1 i1 + i2; the
+ * weights make every stencil offset (o0, o1, o2 triples, npoints of them)
+ * that precedes the centre in C order lower the step, and every one that
+ * follows it raise the step, so each step reads exactly the values the
+ * lexicographic sweep of c/bmgs/relax.cpp:12-40 reads.  GFD_ENOTSUP when
+ * no such weights exist. */
+int gfd_gs_order(const int* offsets, int npoints, int* a0, int* a1);
 
 /* Replaces Operator.get_diagonal_element() -- c/operators.cpp:319-332. */
 double gfd_operator_get_diagonal_element(const gfd_operator* op);

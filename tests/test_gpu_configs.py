@@ -206,7 +206,7 @@ def test_uneven_48_all_problems():
     say(f'  batched: {n} refinement steps, max relres {rel.max():.3g}')
     assert (rel <= 1e-10).all()
     worst = 0.0
-    for i, (gd, (_, rho)) in enumerate(problems):
+    for i, (gd, rho) in enumerate(problems):
         s = FDPoissonSolver(nn=3, eps=1e-24 * float((rho**2).sum()) * gd.dv,
                             maxiter=600)
         s.set_grid_descriptor(gd)
