@@ -38,6 +38,9 @@ SIGNATURES = {
     'gfd_operator_relax': (_i, [_vp, _i, _vp, _vp, _i, _d, _vp]),
     'gfd_operator_get_diagonal_element': (_d, [_vp]),
     'gfd_operator_residual': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _dp, _vp]),
+    'gfd_operator_residual_noalloc': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _dp,
+                                           _vp]),
+    'gfd_operator_residual_noalloc_max_blocks': (_l, []),
     'gfd_gs_order': (_i, [_ip, _i, _ip, _ip]),
     'gfd_transformer_new': (_i, [_lp, _lp, _i, _lp, _lp, _lp, _lp, _i, _i,
                                  ctypes.POINTER(_vp)]),

@@ -459,6 +459,88 @@ extern "C" int gfd_operator_residual(gfd_operator* op, const void* in,
     return OK;
 }
 
+// ------------------------------------------- residual without allocation
+//
+// gfd_operator_residual's norm needs a partial-sum buffer (one double per
+// launched block) and a result slot; the operator grows its own with
+// hipMalloc.  The variant below takes them from one fixed device array
+// instead, so it never allocates: layout [0] the norm, [NA_PARTIAL ..] the
+// block partials.  Every launched block writes its own slot and the final
+// sum writes [0], so nothing has to be cleared between calls.  All calls on
+// one GPU share that GPU's array: calls that overlap in time race.
+constexpr long NA_DOUBLES = 1L << 17;  // 1 MiB per GPU
+constexpr long NA_PARTIAL = 32;        // partials start 256 bytes in
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+constexpr int NA_MAXDEV = 64;
+static std::mutex g_na_mu;
+static double* g_na_ptr[NA_MAXDEV];  // device address of scratch, per GPU
+
+// The current device's address of `scratch`, looked up once per device.
+static int noalloc_scratch(double** out)
+{
+    int dev = -1;
+    GFD_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAXDEV) {
+        set_error("residual_noalloc: device %d beyond the %d supported", dev,
+                  NA_MAXDEV);
+        return EINVAL_;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        GFD_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_na_ptr[dev] = (double*)p;
+    }
+    *out = g_na_ptr[dev];
+    return OK;
+}
+
+extern "C" long gfd_operator_residual_noalloc_max_blocks(void)
+{
+    return NA_DOUBLES - NA_PARTIAL;
+}
+
+extern "C" int gfd_operator_residual_noalloc(gfd_operator* op,
+                                             const void* in, const void* b,
+                                             void* r, long nin,
+                                             const double* phases,
+                                             double* norm2, void* stream)
+{
+    if (!op || nin < 0 || (nin > 0 && (!in || !b || !r)) || in == r) {
+        set_error("gfd_operator_residual_noalloc: bad arguments");
+        return EINVAL_;
+    }
+    if (nin == 0) {
+        if (norm2) *norm2 = 0.0;
+        return OK;
+    }
+    const int dtype = op->dtype();
+    const long nbmax = op_max_blocks(op, dtype, nin);
+    if (norm2 && nbmax > NA_DOUBLES - NA_PARTIAL) {
+        set_error("gfd_operator_residual_noalloc: %ld grids of %d x %d x %d "
+                  "need %ld partial sums, above the fixed %ld; split the "
+                  "batch", nin, op->n[0], op->n[1], op->n[2], nbmax,
+                  NA_DOUBLES - NA_PARTIAL);
+        return EINVAL_;
+    }
+    double* sp = nullptr;
+    if (norm2) GFD_TRY(noalloc_scratch(&sp));
+    hipStream_t s = (hipStream_t)stream;
+    xr_no_tune_here(true);
+    const int st =
+        residual_norm(op, dtype, in, b, r, nin, op->real ? nullptr : phases,
+                      sp, s, false, sp ? sp + NA_PARTIAL : nullptr);
+    xr_no_tune_here(false);
+    GFD_TRY(st);
+    if (norm2) {
+        GFD_CHECK_HIP(hipMemcpyAsync(norm2, sp, sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        GFD_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return OK;
+}
+
 // ================================================================ transfers
 
 extern "C" int gfd_transformer_new(const long size_in[3],

@@ -94,6 +94,9 @@ std::vector<XmTile> xr_candidates(int R, int dtype, const int n[3], bool dma);
 // Timing autotuner of the full-row kernel's tile (on by default; off also
 // forgets the tiles found).
 void xr_set_autotune(bool on);
+// While on (this thread only): no timing runs; a grid the tuner has not
+// seen yet gets the heuristic tile.  For callers that must not allocate.
+void xr_no_tune_here(bool on);
 // Let the tuner try the LDS-DMA kernel (k_xdma) for plain real passes.
 void xr_set_dma(bool on);
 

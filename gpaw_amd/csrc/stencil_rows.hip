@@ -164,6 +164,11 @@ static std::mutex g_tune_mu;
 static std::map<XrKey, XrTuned> g_tuned;
 static bool g_autotune = true;
 static bool g_xdma = true;
+// calls that must not allocate (events included) take a tile the tuner
+// found earlier, or the heuristic one, on this thread
+static thread_local bool t_no_tune = false;
+
+void xr_no_tune_here(bool on) { t_no_tune = on; }
 
 void xr_set_dma(bool on)
 {
@@ -379,6 +384,7 @@ static XmTile xr_pick_tile(const AxisStencil& st, int dtype, const int n[3],
         if (!g_autotune) return heur;
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return choose(it->second);
+        if (t_no_tune) return heur;
         dma = g_xdma;
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

@@ -91,6 +91,21 @@ int gfd_operator_residual(gfd_operator* op, const void* in, const void* b,
                           void* r, long nin, const double* phases,
                           double* norm2, void* stream);
 
+/* No counterpart in the reference.  gfd_operator_residual for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): same arguments and results, but
+ * the norm's partial sums live in a fixed per-GPU device array of 1 MiB
+ * instead of buffers the operator grows, and the tile tuner is not run (a
+ * grid it has not timed yet gets the heuristic tile).  Returns GFD_EINVAL
+ * when the pass would launch more blocks than
+ * gfd_operator_residual_noalloc_max_blocks() (split the batch).  Calls
+ * that overlap in time on one GPU share that array and race. */
+int gfd_operator_residual_noalloc(gfd_operator* op, const void* in,
+                                  const void* b, void* r, long nin,
+                                  const double* phases, double* norm2,
+                                  void* stream);
+long gfd_operator_residual_noalloc_max_blocks(void);
+
 /* ---------------------------------------------------------------- transfers
  * Replaces _gpaw.Transformer(size_in, size_out, k, pad, npad, skip,
  * neighbors, real, comm, interpolate[, use_gpu]) --
