@@ -99,7 +99,10 @@ def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
     V(2,2)-cycles (Jacobi weight w0 on the fine level, 1 below; the four
     levels 192^3 .. 24^3 as grid-wide launches, 12^3 .. 3^3 in one
     LDS-resident workgroup with `coarse` sweeps on 3^3; `inner` cycles per
-    step) inside FP64 residual refinement."""
+    step) inside FP64 residual refinement.  The same refinement with the
+    inner cycles in FP64 (fp64_same_schedule) isolates what FP32 gains;
+    `levels4` repeats both on the four-level cycles alone (64 sweeps on
+    24^3 in place of the coarser levels)."""
     import torch
     from gpaw_amd import workloads
     from gpaw_amd.poisson import FDPoissonSolver
@@ -124,17 +127,29 @@ def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
         ms, n64 = wall(lambda: s.solve_neutral(phi64, drho))
         t64.append(ms)
     rel64 = (s.last_error / (rn2 * gd.dv))**0.5
-    phi = torch.zeros_like(drho)
-    kw = dict(pre=2, post=2, inner=inner, coarse_sweeps=coarse, w0=w0)
-    s.solve_mixed(phi, drho, tol, **kw)
-    tmx = []
-    for _ in range(3):
-        phi.zero_()
-        ms, (nmx, relmx) = wall(lambda: s.solve_mixed(phi, drho, tol, **kw))
-        tmx.append(ms)
+    def refine(levels, fp64_inner, cs=coarse):
+        """best-of-3 time of the refinement solve on one schedule"""
+        ph = torch.zeros_like(drho)
+        kw = dict(levels=levels, pre=2, post=2, inner=inner,
+                  coarse_sweeps=cs, w0=w0, fp64_inner=fp64_inner)
+        s.solve_mixed(ph, drho, tol, **kw)
+        t = []
+        for _ in range(3):
+            ph.zero_()
+            ms, (n, rel) = wall(lambda: s.solve_mixed(ph, drho, tol, **kw))
+            t.append(ms)
+        return min(t), n, rel, ph
+
+    tmx, nmx, relmx, phi = refine(None, False)
+    t64s, n64s, rel64s, phi64s = refine(None, True)
+    # the configuration's 4-level cycles (coarsest 24^3: 64 sweeps there)
+    t4, n4, rel4, phi4 = refine(4, False, 64)
+    t4d, n4d, rel4d, _ = refine(4, True, 64)
     a = phi - phi.mean()
     b = phi64 - phi64.mean()
     err = float((a - b).abs().max() / b.abs().max())
+    c4 = phi4 - phi4.mean()
+    err4 = float((c4 - b).abs().max() / b.abs().max())
     # the FFT solver on the same problem (exact for the FD operator)
     from math import pi
     from gpaw_amd.fd_operators import Laplace
@@ -153,10 +168,24 @@ def stage_poisson_192(dev, tol=1e-10, inner=2, coarse=8, w0=0.8):
     c = pf - pf.mean()
     errf = float((c - b).abs().max() / b.abs().max())
     return {'fp64_ms': round(min(t64), 3), 'fp64_vcycles': n64,
-            'fp64_relres': rel64, 'mixed_ms': round(min(tmx), 3),
+            'fp64_relres': rel64, 'mixed_ms': round(tmx, 3),
             'mixed_refinements': nmx, 'mixed_relres': relmx,
-            'speedup_mixed_vs_fp64': round(min(t64) / min(tmx), 3),
+            'speedup_mixed_vs_fp64': round(min(t64) / tmx, 3),
             'mixed_err_vs_fp64': err,
+            'fp64_same_schedule_ms': round(t64s, 3),
+            'fp64_same_schedule_refinements': n64s,
+            'fp64_same_schedule_relres': rel64s,
+            'speedup_precision_alone': round(t64s / tmx, 3),
+            'levels4': {'mixed_ms': round(t4, 3), 'refinements': n4,
+                        'relres': rel4, 'err_vs_fp64': err4,
+                        'fp64_same_schedule_ms': round(t4d, 3),
+                        'fp64_same_schedule_refinements': n4d,
+                        'fp64_same_schedule_relres': rel4d,
+                        'speedup_precision_alone': round(t4d / t4, 3),
+                        'speedup_vs_fp64_reference_schedule':
+                            round(min(t64) / t4, 3),
+                        'cycle': 'V(2,2) over 4 levels (192^3 .. 24^3), '
+                                 '64 sweeps on 24^3'},
             'fft_ms': round(min(tf), 3), 'fft_relres': relf,
             'fft_err_vs_fp64': errf,
             'levels': s.levels + 1,
@@ -216,6 +245,22 @@ def stage_uneven_48(dev, tol=1e-10):
 
     run64()
     ms64, its = wall(run64)
+
+    # the batched schedule on one grid at a time: FP32 inner cycles
+    # (batching alone) and FP64 inner cycles (precision alone)
+    def run_seq(fp64_inner):
+        ps = [torch.zeros_like(r) for r in rhos]
+        ns = []
+        for s, r, p in zip(solvers, rhos, ps):
+            n, _ = s.solve_mixed(p, r, tol, pre=2, post=2, coarse_sweeps=64,
+                                 w0=0.8, inner=1, fp64_inner=fp64_inner)
+            ns.append(n)
+        return ns
+
+    run_seq(False)
+    ms_seq32, ns32 = wall(lambda: run_seq(False))
+    run_seq(True)
+    ms_seq64, ns64 = wall(lambda: run_seq(True))
     err = 0.0
     for g, p, q in zip(gds, phis, refs):
         a, c = p, q
@@ -227,8 +272,58 @@ def stage_uneven_48(dev, tol=1e-10):
             'fp64_sequential_ms': round(ms64, 2),
             'fp64_vcycles_total': int(sum(its)),
             'speedup': round(ms64 / best, 2), 'max_err_vs_fp64': err,
+            'same_schedule_sequential_fp32_ms': round(ms_seq32, 2),
+            'same_schedule_sequential_fp64_ms': round(ms_seq64, 2),
+            'same_schedule_refinements_fp32_fp64': [int(sum(ns32)),
+                                                    int(sum(ns64))],
+            'speedup_batching_alone': round(ms_seq32 / best, 2),
+            'speedup_precision_alone_sequential': round(ms_seq64 / ms_seq32,
+                                                        2),
+            'preconditioner': stage_uneven_48_precond(dev, gds),
             'problems': len(gds), 'points': npts,
             'tiles': {str(k): v for k, v in b.tiles().items()}}
+
+
+def stage_uneven_48_precond(dev, gds, reps=10):
+    """The preconditioner half of the 48-problem workload: the FP32 chain
+    on every grid in one set of launches (BatchedPreconditioner) against
+    the single-grid FP32 preconditioner called grid by grid.  Traffic model
+    per fine point: FP64 residual read twice (restriction, last pass), FP64
+    result written once, FP32 q0 written and read once (40 B), plus the
+    coarse levels (~1/8 and 1/64 of that)."""
+    import torch
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import BatchedPreconditioner, Preconditioner
+    rng = np.random.default_rng(4807)
+    res = [torch.from_numpy(rng.uniform(-1, 1, tuple(g.n_c))).to(dev)
+           for g in gds]
+    npts = sum(int(np.prod(g.n_c)) for g in gds)
+    P = BatchedPreconditioner(gds)
+    out = P(res)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = P(res, out)
+    torch.cuda.synchronize()
+    ms_b = (time.perf_counter() - t0) * 1e3 / reps
+    singles = [Preconditioner(g, Laplace(g, -0.5, 3), float, fp32=True)
+               for g in gds]
+    outs = [p(r, None) for p, r in zip(singles, res)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs = [p(r, None) for p, r in zip(singles, res)]
+    torch.cuda.synchronize()
+    ms_s = (time.perf_counter() - t0) * 1e3 / reps
+    diff = max(float((a - b).abs().max() / b.abs().max())
+               for a, b in zip(out, outs))
+    gb = npts * 40 * (1 + 1 / 8 + 1 / 64) / 1e9
+    return {'batched_ms': round(ms_b, 3), 'sequential_fp32_ms': round(ms_s, 3),
+            'speedup_batching': round(ms_s / ms_b, 2),
+            'GPts/s': round(npts / ms_b / 1e6, 1),
+            'model_GB': round(gb, 3),
+            'hbm_frac_model': round(gb / ms_b / 8.0, 4),
+            'max_diff_vs_single_grid': diff}
 
 
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):

@@ -56,7 +56,7 @@ SIGNATURES = {
     'gfd_poisson_solve': (_i, [_vp, _vp, _vp, _d, _i, _i, _ip, _dp, _vp]),
     'gfd_poisson_set_graph': (_i, [_vp, _i]),
     'gfd_poisson_solve_mixed': (_i, [_vp, _vp, _vp, _d, _i, _i, _i, _i, _d,
-                                     _d, _i, _i, _ip, _dp, _vp]),
+                                     _d, _i, _i, _i, _ip, _dp, _vp]),
     'gfd_batch_poisson_new': (_i, [_i, _i, ctypes.POINTER(_vp),
                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                    _dp, ctypes.POINTER(_vp)]),
@@ -74,6 +74,7 @@ SIGNATURES = {
     'gfd_batch_poisson_solve_mixed': (_i, [_vp, ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), _d, _i, _i,
                                            _i, _i, _i, _ip, _dp, _vp]),
+    'gfd_batch_precond32': (_i, [_vp, _dp, _vp, _vp, _vp]),
     'gfd_fft_poisson_spectrum_shape': (_l, [ctypes.POINTER(_l), _ip,
                                             ctypes.POINTER(_l)]),
     'gfd_fft_poisson_new': (_i, [ctypes.POINTER(_l), _ip, _dp,

@@ -846,6 +846,8 @@ struct gfd_poisson {
     double dv = 1.0;
     Levels lv;     // f64 levels (phi[0], rho[0]: owned copies for graphs)
     Levels lv32;   // f32 levels of the mixed-precision inner cycles
+    Levels lv64r;  // f64 correction levels: refinement with FP64 inner
+                   // cycles on the mixed solve's schedule
     DevBuf scalar; // norms
     // partial sums of the norms and work space of K > 2 transfers: owned
     // here (not borrowed from the level operators / transformers, which
@@ -860,7 +862,10 @@ struct gfd_poisson {
     hipGraphExec_t g32 = nullptr;
     int g32_key[6] = {-1, -1, -1, -1, -1, -1};
     double g32_w[2] = {0, 0};
-    CoarseSolve cs64, cs32;
+    hipGraphExec_t g64r = nullptr;
+    int g64r_key[6] = {-1, -1, -1, -1, -1, -1};
+    double g64r_w[2] = {0, 0};
+    CoarseSolve cs64, cs32, cs64r;
 };
 
 static int alloc_levels(gfd_poisson* p, Levels& lv, int dtype, int L)
@@ -960,8 +965,10 @@ static void drop_graphs(gfd_poisson* p)
 {
     if (p->g64) (void)hipGraphExecDestroy(p->g64);
     if (p->g32) (void)hipGraphExecDestroy(p->g32);
+    if (p->g64r) (void)hipGraphExecDestroy(p->g64r);
     p->g64 = nullptr;
     p->g32 = nullptr;
+    p->g64r = nullptr;
 }
 
 extern "C" void gfd_poisson_free(gfd_poisson* p)
@@ -973,6 +980,7 @@ extern "C" void gfd_poisson_free(gfd_poisson* p)
     if (p->gs) (void)hipStreamDestroy(p->gs);
     p->lv.pool.release();
     p->lv32.pool.release();
+    p->lv64r.pool.release();
     p->scalar.release();
     p->partial.release();
     p->xscratch.release();
@@ -1142,7 +1150,8 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
                         xe, s, p->xscratch.p));
         void* c1 = lv.phi[l + 1];
         void* a1 = lv.tmp[l + 1];
-        const CoarseSolve& cs = dt == 2 ? p->cs32 : p->cs64;
+        const CoarseSolve& cs =
+            &lv == &p->lv64r ? p->cs64r : (dt == 2 ? p->cs32 : p->cs64);
         if (cs.valid && cs.lc == l + 1) {
             // levels l+1 .. coarsest in one LDS-resident workgroup
             GFD_TRY(launch_coarse((const CGrid*)cs.dev.p, 1, cs.lds, dt,
@@ -1222,6 +1231,7 @@ static int capture(gfd_poisson* p, hipGraphExec_t* exec, F fn)
 {
     GFD_TRY(pretune_levels(p, p->lv, p->gs));
     if (p->lv32.L >= 0) GFD_TRY(pretune_levels(p, p->lv32, p->gs));
+    if (p->lv64r.L >= 0) GFD_TRY(pretune_levels(p, p->lv64r, p->gs));
     hipGraph_t g = nullptr;
     GFD_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
     int st = fn(p->gs);
@@ -1381,22 +1391,27 @@ extern "C" int gfd_poisson_set_graph(gfd_poisson* p, int on)
 // One refinement step of the mixed-precision solve on the owned buffers:
 // e = 0; `inner` f32 V-cycles on L e = r32; phi -= e; r32 = (float)(L phi
 // - rho) with the FP64 squared norm into *dn.
+// One refinement step: e from `inner` V-cycles on L e = r (FP32 levels, or
+// FP64 with fp64), phi -= e, then the new FP64 residual r and its norm.
 static int mixed_step(gfd_poisson* p, const Schedule& sc, int inner,
-                      double* dn, hipStream_t s)
+                      double* dn, hipStream_t s, bool fp64)
 {
     const long ng = p->ops[0]->ng();
     double* phi0 = (double*)p->lv.phi[0];
     const double* rho0 = (const double*)p->lv.rho[0];
-    Levels& lv = p->lv32;
-    float* r32 = (float*)lv.rho[0];
+    Levels& lv = fp64 ? p->lv64r : p->lv32;
+    void* r0 = lv.rho[0];
     void* cur = lv.phi[0];
     void* alt = lv.tmp[0];
-    GFD_TRY(vec_zero_bytes(cur, 4 * ng, s));
+    GFD_TRY(vec_zero_bytes(cur, (fp64 ? 8 : 4) * ng, s));
     for (int c = 0; c < inner; c++)
-        GFD_TRY(vcycle(p, lv, sc, 0, &cur, &alt, r32, s));
-    GFD_TRY(vec_sub_f(phi0, (const float*)cur, ng, s));
-    return residual_norm(p->ops[0], 0, phi0, rho0, r32, 1, nullptr, dn, s,
-                         true, (double*)p->partial.p);
+        GFD_TRY(vcycle(p, lv, sc, 0, &cur, &alt, r0, s));
+    if (fp64)
+        GFD_TRY(vec_sub_d(phi0, (const double*)cur, ng, s));
+    else
+        GFD_TRY(vec_sub_f(phi0, (const float*)cur, ng, s));
+    return residual_norm(p->ops[0], 0, phi0, rho0, r0, 1, nullptr, dn, s,
+                         !fp64, (double*)p->partial.p);
 }
 
 extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
@@ -1404,8 +1419,8 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
                                        int levels, int pre, int post,
                                        int coarse_sweeps, double w0,
                                        double wc, int inner, int max_outer,
-                                       int* n_outer, double* relres,
-                                       void* stream)
+                                       int inner_fp64, int* n_outer,
+                                       double* relres, void* stream)
 {
     if (!p || !phi || !rho || !n_outer || max_outer < 1 || inner < 1 ||
         levels < 1 || pre < 0 || post < 0) {
@@ -1413,8 +1428,14 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
         return EINVAL_;
     }
     hipStream_t s = (hipStream_t)stream;
+    const bool fp64 = inner_fp64 != 0;
     const int L32 = std::min(levels - 1, p->L);
-    GFD_TRY(alloc_levels(p, p->lv32, 2, L32));
+    Levels& lvi = fp64 ? p->lv64r : p->lv32;
+    CoarseSolve& csi = fp64 ? p->cs64r : p->cs32;
+    hipGraphExec_t& gi = fp64 ? p->g64r : p->g32;
+    int* gkey = fp64 ? p->g64r_key : p->g32_key;
+    double* gw = fp64 ? p->g64r_w : p->g32_w;
+    GFD_TRY(alloc_levels(p, lvi, fp64 ? 0 : 2, L32));
     GFD_TRY(p->scalar.ensure(4 * sizeof(double), s));
     double* dn = (double*)p->scalar.p;
     Schedule sc = ref_schedule(p);
@@ -1425,14 +1446,13 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
     sc.w0 = w0;
     sc.wc = wc;
     bool cs_changed = false;
-    GFD_TRY(plan_coarse(p, sc, 2, p->cs32, s, &cs_changed));
+    GFD_TRY(plan_coarse(p, sc, fp64 ? 0 : 2, csi, s, &cs_changed));
     const int key[6] = {L32, pre, post, coarse_sweeps, inner, 0};
-    bool same = p->g32 && !cs_changed && p->g32_w[0] == w0 &&
-                p->g32_w[1] == wc;
-    for (int i = 0; i < 6; i++) same = same && p->g32_key[i] == key[i];
-    if (!same && p->g32) {
-        (void)hipGraphExecDestroy(p->g32);
-        p->g32 = nullptr;
+    bool same = gi && !cs_changed && gw[0] == w0 && gw[1] == wc;
+    for (int i = 0; i < 6; i++) same = same && gkey[i] == key[i];
+    if (!same && gi) {
+        (void)hipGraphExecDestroy(gi);
+        gi = nullptr;
     }
     GFD_TRY(enter_owned(p, phi, rho, s));
     const long ng = p->ops[0]->ng();
@@ -1442,24 +1462,24 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
     double rn2 = 0.0, err2 = 0.0;
     GFD_TRY(vec_dot(rho0, rho0, ng, (double*)p->partial.p, dn, p->gs));
     GFD_TRY(read_scalar(p, p->gs, &rn2));
-    GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, p->lv32.rho[0], 1,
-                          nullptr, dn, p->gs, true, (double*)p->partial.p));
+    GFD_TRY(residual_norm(p->ops[0], 0, phi0, rho0, lvi.rho[0], 1, nullptr,
+                          dn, p->gs, !fp64, (double*)p->partial.p));
     GFD_TRY(read_scalar(p, p->gs, &err2));
     const double tol2 = tol_rel * tol_rel * rn2;
     int outer = 0;
     while (err2 > tol2 && outer < max_outer) {
         if (p->use_graph) {
-            if (!p->g32) {
-                GFD_TRY(capture(p, &p->g32, [&](hipStream_t cs) {
-                    return mixed_step(p, sc, inner, dn, cs);
+            if (!gi) {
+                GFD_TRY(capture(p, &gi, [&](hipStream_t cs) {
+                    return mixed_step(p, sc, inner, dn, cs, fp64);
                 }));
-                for (int i = 0; i < 6; i++) p->g32_key[i] = key[i];
-                p->g32_w[0] = w0;
-                p->g32_w[1] = wc;
+                for (int i = 0; i < 6; i++) gkey[i] = key[i];
+                gw[0] = w0;
+                gw[1] = wc;
             }
-            GFD_CHECK_HIP(hipGraphLaunch(p->g32, p->gs));
+            GFD_CHECK_HIP(hipGraphLaunch(gi, p->gs));
         } else {
-            GFD_TRY(mixed_step(p, sc, inner, dn, p->gs));
+            GFD_TRY(mixed_step(p, sc, inner, dn, p->gs, fp64));
         }
         GFD_TRY(read_scalar(p, p->gs, &err2));
         outer++;

@@ -40,7 +40,8 @@ struct BSt {
     int tz, ty;        // tiles along z and y
     int xc;            // x planes per tile (tiles = ceil(n0 / xc) ty tz)
     int has[3];
-    double c0, w;
+    double c0, w;      // w: Jacobi weight
+    double step;       // step of EPI_UPDATE / EPI_UPDW
     double cp[3][BMAXR], cm[3][BMAXR];
 };
 
@@ -52,6 +53,7 @@ struct BSArgs {
     void* out;
     const void* aux;
     double* partial;   // one double per tile (EPI_RESID)
+    int negate;        // EPI_UPDATE / EPI_UPDW: store the negated result
 };
 
 __device__ inline int bfind_tile(const BSt* g, int G, long t)
@@ -78,8 +80,11 @@ __device__ inline bool bwrap(int& i, int n, int periodic)
 }
 
 // T: data read, TO: data written (float output of an FP64 residual).
-// EPI_PLAIN, EPI_RELAX (aux = right-hand side) or EPI_RESID (aux = rhs,
-// partial[tile] = sum r^2).  A tile is BZ x PZ z points by BY rows of one
+// EPI_PLAIN, EPI_RELAX (aux = right-hand side), EPI_RESID (aux = rhs,
+// partial[tile] = sum r^2, when partial is set), EPI_UPDATE
+// (out = +-(v - step (L v - aux)), the preconditioner's smoothing step) or
+// EPI_UPDW (EPI_UPDATE on FP32 data with FP64 aux and out: aux narrowed
+// to FP32 first, the FP32 result widened).  A tile is BZ x PZ z points by BY rows of one
 // grid and marches over an x chunk: the centre values of planes x-R .. x+R
 // stay in a register queue, the z and y neighbours come through L1 / L2.
 template <typename T, typename TO, int R, int PZ, int EPI>
@@ -227,6 +232,16 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
                         const T f = ((const T*)a.aux)[obase + p];
                         r = ((S)1.0 - (S)d.w) * v +
                             (S)d.w * (f - acc[p]) / (S)d.c0;
+                    } else if constexpr (EPI == EPI_UPDATE ||
+                                         EPI == EPI_UPDW) {
+                        T f;
+                        if constexpr (EPI == EPI_UPDW)
+                            f = (T)((const double*)a.aux)[obase + p];
+                        else
+                            f = ((const T*)a.aux)[obase + p];
+                        const T q = fmac(acc[p], (S)d.c0, v) - f;
+                        r = fmac(v, -(S)d.step, q);
+                        if (a.negate) r = -r;
                     } else {
                         const T f = ((const T*)a.aux)[obase + p];
                         r = fmac(acc[p], (S)d.c0, v) - f;
@@ -280,6 +295,7 @@ struct BXf {
     int ni[3], no[3];
     int off[3], skip0[3];
     int periodic[3];
+    double alpha;      // XEPI_SCALE2: out2 = alpha out (per grid)
 };
 
 struct BXArgs {
@@ -288,7 +304,9 @@ struct BXArgs {
     long total;
     const void* in;
     void* out;
-    const void* base;  // interpolation: out = base - I(in)
+    const void* base;  // XEPI_SUBFROM: out = base - T(in)
+    void* out2;        // XEPI_SCALE2
+    int negate;        // XEPI_SCALE2: out = -T(in)
 };
 
 __device__ inline int bfind_point(const BXf* g, int G, long p)
@@ -304,21 +322,25 @@ __device__ inline int bfind_point(const BXf* g, int G, long p)
     return lo;
 }
 
-template <typename T>
-__device__ inline T bxload(const BXf& d, const T* __restrict__ g, int i0,
+// input value (type TI, narrowed to T) at a wrapped index; 0 outside a
+// zero boundary
+template <typename T, typename TI>
+__device__ inline T bxload(const BXf& d, const TI* __restrict__ g, int i0,
                            int i1, int i2)
 {
     if (!bwrap(i0, d.ni[0], d.periodic[0]) ||
         !bwrap(i1, d.ni[1], d.periodic[1]) ||
         !bwrap(i2, d.ni[2], d.periodic[2]))
         return zero_of<T>();
-    return g[((long)i0 * d.ni[1] + i1) * d.ni[2] + i2];
+    return (T)g[((long)i0 * d.ni[1] + i1) * d.ni[2] + i2];
 }
 
-// Restriction (INTERP false) or interpolation with out = base - I(in)
-// (INTERP true), K = 2 weights, the association order of the single-grid
-// fused kernels (z innermost, then y, then x).
-template <typename T, bool INTERP>
+// Restriction (INTERP false) or interpolation (INTERP true) with K = 2
+// weights, the association order of the single-grid fused kernels (z
+// innermost, then y, then x); input of type TI (FP64 residuals into an
+// FP32 chain) narrowed on load; result T stored by the transfer epilogue
+// XE (transfer.h).
+template <typename T, typename TI, bool INTERP, int XE>
 __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
 {
     typedef typename ScalarOf<T>::type S;
@@ -336,7 +358,7 @@ __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
         r /= d.no[2];
         const int o1 = (int)(r % d.no[1]);
         const int o0 = (int)(r / d.no[1]);
-        const T* __restrict__ g = (const T*)a.in + d.in_off;
+        const TI* __restrict__ g = (const TI*)a.in + d.in_off;
         const long oidx =
             d.out_off + ((long)o0 * d.no[1] + o1) * d.no[2] + o2;
         T v;
@@ -349,16 +371,15 @@ __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
                 T py[3];
 #pragma unroll
                 for (int dy = -1; dy <= 1; dy++) {
-                    const T zp = bxload(d, g, c0 + dx, c1 + dy, c2 + 1) +
-                                 bxload(d, g, c0 + dx, c1 + dy, c2 - 1);
-                    py[dy + 1] = (S)0.5 * fmac(bxload(d, g, c0 + dx, c1 + dy,
+                    const T zp = bxload<T, TI>(d, g, c0 + dx, c1 + dy, c2 + 1) +
+                                 bxload<T, TI>(d, g, c0 + dx, c1 + dy, c2 - 1);
+                    py[dy + 1] = (S)0.5 * fmac(bxload<T, TI>(d, g, c0 + dx, c1 + dy,
                                                       c2),
                                                (S)0.5, zp);
                 }
                 px[dx + 1] = (S)0.5 * fmac(py[1], (S)0.5, py[2] + py[0]);
             }
             v = (S)0.5 * fmac(px[1], (S)0.5, px[2] + px[0]);
-            ((T*)a.out)[oidx] = v;
         } else {
             const int q0 = o0 + d.skip0[0], q1 = o1 + d.skip0[1],
                       q2 = o2 + d.skip0[2];
@@ -373,15 +394,25 @@ __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
 #pragma unroll
                 for (int dy = 0; dy < 2; dy++) {
                     if (dy >= m1) break;
-                    T vz = bxload(d, g, c0 + dx, c1 + dy, c2);
+                    T vz = bxload<T, TI>(d, g, c0 + dx, c1 + dy, c2);
                     if (m2 == 2)
                         vz = (S)0.5 *
-                             (vz + bxload(d, g, c0 + dx, c1 + dy, c2 + 1));
+                             (vz + bxload<T, TI>(d, g, c0 + dx, c1 + dy, c2 + 1));
                     vy[dy] = vz;
                 }
                 vx[dx] = (m1 == 2) ? (S)0.5 * (vy[0] + vy[1]) : vy[0];
             }
             v = (m0 == 2) ? (S)0.5 * (vx[0] + vx[1]) : vx[0];
+        }
+        if constexpr (XE == XEPI_PLAIN) {
+            ((T*)a.out)[oidx] = v;
+        } else if constexpr (XE == XEPI_NEG) {
+            ((T*)a.out)[oidx] = -v;
+        } else if constexpr (XE == XEPI_SCALE2) {
+            if (a.negate) v = -v;
+            ((T*)a.out)[oidx] = v;
+            ((T*)a.out2)[oidx] = (S)d.alpha * v;
+        } else {
             ((T*)a.out)[oidx] = ((const T*)a.base)[oidx] - v;
         }
     }
@@ -472,7 +503,8 @@ static int upload_level(BLevel& lv, hipStream_t s)
 
 template <typename T, typename TO, int R, int PZ, int EPI>
 static int bst_launch_t(const BLevel& lv, const void* in, void* out,
-                        const void* aux, double* partial, hipStream_t s)
+                        const void* aux, double* partial, hipStream_t s,
+                        int negate)
 {
     BSArgs a;
     a.g = (const BSt*)lv.d_st.p;
@@ -482,6 +514,7 @@ static int bst_launch_t(const BLevel& lv, const void* in, void* out,
     a.out = out;
     a.aux = aux;
     a.partial = partial;
+    a.negate = negate;
     long nb = lv.tile.blocks > 0 ? std::min(lv.tile.blocks, lv.ntiles)
                                  : lv.ntiles;
     if (nb <= 0) return OK;
@@ -494,11 +527,13 @@ static int bst_launch_t(const BLevel& lv, const void* in, void* out,
 
 template <typename T, typename TO, int EPI>
 static int bst_launch_r(const BLevel& lv, const void* in, void* out,
-                        const void* aux, double* partial, hipStream_t s)
+                        const void* aux, double* partial, hipStream_t s,
+                        int negate = 0)
 {
 #define GFD_BST(RR, PP)                                                       \
     if (lv.R == RR && lv.tile.pz == PP)                                       \
-        return bst_launch_t<T, TO, RR, PP, EPI>(lv, in, out, aux, partial, s);
+        return bst_launch_t<T, TO, RR, PP, EPI>(lv, in, out, aux, partial, s, \
+                                                negate);
     GFD_BST(1, 1) GFD_BST(1, 2) GFD_BST(1, 4)
     GFD_BST(2, 1) GFD_BST(2, 2) GFD_BST(2, 4)
     GFD_BST(3, 1) GFD_BST(3, 2) GFD_BST(3, 4)
@@ -508,9 +543,11 @@ static int bst_launch_r(const BLevel& lv, const void* in, void* out,
     return ENOTSUP_;
 }
 
-// dtype 0: f64 -> f64, 2: f32 -> f32, 4: f64 -> f32 (RESID only)
+// dtype 0: f64 -> f64, 2: f32 -> f32 (EPI_UPDW: f32 -> f64), 4: f64 ->
+// f32 (RESID only)
 static int bst(const BLevel& lv, int dtype, int epi, const void* in,
-               void* out, const void* aux, double* partial, hipStream_t s)
+               void* out, const void* aux, double* partial, hipStream_t s,
+               int negate = 0)
 {
     if (dtype == 0 && epi == EPI_PLAIN)
         return bst_launch_r<double, double, EPI_PLAIN>(lv, in, out, aux,
@@ -527,13 +564,30 @@ static int bst(const BLevel& lv, int dtype, int epi, const void* in,
     if (dtype == 4 && epi == EPI_RESID)
         return bst_launch_r<double, float, EPI_RESID>(lv, in, out, aux,
                                                       partial, s);
+    if (dtype == 2 && epi == EPI_UPDATE)
+        return bst_launch_r<float, float, EPI_UPDATE>(lv, in, out, aux,
+                                                      partial, s, negate);
+    if (dtype == 2 && epi == EPI_UPDW)
+        return bst_launch_r<float, double, EPI_UPDW>(lv, in, out, aux,
+                                                     partial, s, negate);
     set_error("batched stencil: dtype %d / epilogue %d not built", dtype,
               epi);
     return ENOTSUP_;
 }
 
+template <typename T, typename TI, bool I, int XE>
+static void bxfer_launch(const BXArgs& a, long nb, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bxfer2<T, TI, I, XE>), dim3((unsigned)nb),
+                       dim3(256), 0, s, a);
+}
+
+// dtype 0: FP64, 2: FP32, 4: FP32 from FP64 input (restriction).
+// Restriction epilogue xe (PLAIN, SCALE2 with the grids' alpha); the
+// interpolation stores base - I(in) (SUBFROM) or -I(in) (NEG).
 static int bxfer(const BPair& pr, bool interp, int dtype, const void* in,
-                 void* out, const void* base, int num_cu, hipStream_t s)
+                 void* out, const void* base, int num_cu, hipStream_t s,
+                 int xe = -1, void* out2 = nullptr, int negate = 0)
 {
     BXArgs a;
     a.g = (const BXf*)(interp ? pr.d_itp.p : pr.d_rst.p);
@@ -542,22 +596,29 @@ static int bxfer(const BPair& pr, bool interp, int dtype, const void* in,
     a.in = in;
     a.out = out;
     a.base = base;
+    a.out2 = out2;
+    a.negate = negate;
+    if (xe < 0) xe = interp ? XEPI_SUBFROM : XEPI_PLAIN;
     if (a.total <= 0) return OK;
     const long nb = std::min((a.total + 255) / 256, (long)num_cu * 16);
-    if (dtype == 0) {
-        if (interp)
-            hipLaunchKernelGGL((k_bxfer2<double, true>), dim3((unsigned)nb),
-                               dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_bxfer2<double, false>), dim3((unsigned)nb),
-                               dim3(256), 0, s, a);
-    } else {
-        if (interp)
-            hipLaunchKernelGGL((k_bxfer2<float, true>), dim3((unsigned)nb),
-                               dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_bxfer2<float, false>), dim3((unsigned)nb),
-                               dim3(256), 0, s, a);
+    if (dtype == 0 && !interp && xe == XEPI_PLAIN)
+        bxfer_launch<double, double, false, XEPI_PLAIN>(a, nb, s);
+    else if (dtype == 0 && interp && xe == XEPI_SUBFROM)
+        bxfer_launch<double, double, true, XEPI_SUBFROM>(a, nb, s);
+    else if (dtype == 2 && !interp && xe == XEPI_PLAIN)
+        bxfer_launch<float, float, false, XEPI_PLAIN>(a, nb, s);
+    else if (dtype == 2 && !interp && xe == XEPI_SCALE2)
+        bxfer_launch<float, float, false, XEPI_SCALE2>(a, nb, s);
+    else if (dtype == 4 && !interp && xe == XEPI_SCALE2)
+        bxfer_launch<float, double, false, XEPI_SCALE2>(a, nb, s);
+    else if (dtype == 2 && interp && xe == XEPI_SUBFROM)
+        bxfer_launch<float, float, true, XEPI_SUBFROM>(a, nb, s);
+    else if (dtype == 2 && interp && xe == XEPI_NEG)
+        bxfer_launch<float, float, true, XEPI_NEG>(a, nb, s);
+    else {
+        set_error("batched transfer: dtype %d / epilogue %d not built", dtype,
+                  xe);
+        return ENOTSUP_;
     }
     GFD_LAUNCH_CHECK();
     return OK;
@@ -1253,4 +1314,75 @@ extern "C" int gfd_batch_poisson_solve_mixed(
         return ECONVERGE;
     }
     return OK;
+}
+
+// ------------------------------------------------- batched preconditioner
+
+// The FP32 kinetic preconditioner chain of gfd_fd_precond32
+// (c/fd_preconditioner.cpp:221-345) on every grid of a three-level batch
+// at once: eight persistent launches, each over all grids.  Work arrays
+// (level 0: q0; levels 1, 2: r, d, q) are the batch's FP32 level arrays.
+// steps[g] is grid g's 0.66666666 / diag(K0); levels 1 and 2 use 4 and 16
+// times it, as the reference does.
+extern "C" int gfd_batch_precond32(gfd_batch_poisson* b, const double* steps,
+                                   const double* res, double* out,
+                                   void* stream)
+{
+    if (!b || !steps || !res || !out || (const void*)res == (void*)out) {
+        set_error("gfd_batch_precond32: bad arguments");
+        return EINVAL_;
+    }
+    if (b->L != 3) {
+        set_error("gfd_batch_precond32: the batch needs exactly 3 levels "
+                  "(has %d)", b->L);
+        return EINVAL_;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    bool changed = false;
+    for (int g = 0; g < b->G; g++)
+        if (!std::isfinite(steps[g]) || steps[g] <= 0) {
+            set_error("gfd_batch_precond32: step %d must be positive", g);
+            return EINVAL_;
+        }
+    for (int g = 0; g < b->G; g++) {
+        const double st = steps[g];
+        if (b->lev[0].st[g].step != st) changed = true;
+        b->lev[0].st[g].step = st;
+        b->lev[1].st[g].step = 4 * st;
+        b->lev[2].st[g].step = 16 * st;
+        b->pair[0].rst[g].alpha = 4 * st;
+        b->pair[1].rst[g].alpha = 16 * st;
+    }
+    if (!b->tuned) {
+        GFD_TRY(b_tune(b, s));  // uploads the level and transfer tables
+    } else if (changed) {
+        GFD_TRY(b_alloc(b, s));
+        for (auto& lv : b->lev) GFD_TRY(upload_level(lv, s));
+    }
+    void* q0 = b->e32[0].p;
+    void* r1 = b->r32[1].p;
+    void* d1 = b->e32[1].p;
+    void* q1 = b->t32[1].p;
+    void* r2 = b->r32[2].p;
+    void* d2 = b->e32[2].p;
+    void* q2 = b->t32[2].p;
+    const int cu = b->num_cu;
+    // r1 = R0(-res) (narrowed), d1 = 4 step r1
+    GFD_TRY(bxfer(b->pair[0], false, 4, res, r1, nullptr, cu, s, XEPI_SCALE2,
+                  d1, 1));
+    // q1 = K1 d1 - r1
+    GFD_TRY(bst(b->lev[1], 2, EPI_RESID, d1, q1, r1, nullptr, s));
+    // r2 = R1(q1), d2 = 16 step r2
+    GFD_TRY(bxfer(b->pair[1], false, 2, q1, r2, nullptr, cu, s, XEPI_SCALE2,
+                  d2, 0));
+    // q2 = d2 - 16 step (K2 d2 - r2)
+    GFD_TRY(bst(b->lev[2], 2, EPI_UPDATE, d2, q2, r2, nullptr, s));
+    // d1 -= I2(q2)
+    GFD_TRY(bxfer(b->pair[1], true, 2, q2, d1, d1, cu, s, XEPI_SUBFROM));
+    // q1 = d1 - 4 step (K1 d1 - r1)
+    GFD_TRY(bst(b->lev[1], 2, EPI_UPDATE, d1, q1, r1, nullptr, s));
+    // q0 = -I1(q1)
+    GFD_TRY(bxfer(b->pair[0], true, 2, q1, q0, nullptr, cu, s, XEPI_NEG));
+    // out = -(q0 - step (K0 q0 - res)), FP64 residual and result
+    return bst(b->lev[0], 2, EPI_UPDW, q0, out, res, nullptr, s, 1);
 }

@@ -10,6 +10,7 @@ int vec_copy(const double* a, double* b, long n, hipStream_t s);
 int vec_zero_bytes(void* a, long nbytes, hipStream_t s);  // nbytes % 4 == 0
 int vec_d2f(const double* a, float* b, long n, hipStream_t s);
 int vec_sub_f(double* a, const float* e, long n, hipStream_t s);
+int vec_sub_d(double* a, const double* e, long n, hipStream_t s);  // a -= e
 int vec_add_scalar(double* a, long n, double v, hipStream_t s);
 int vec_copy16(const void* a, void* b, long nbytes, hipStream_t s);
 long vec_dot_partials(long n);

@@ -85,6 +85,15 @@ k_sub_f(double* __restrict__ a, const float* __restrict__ e, long n)
         a[i] -= (double)e[i];
 }
 
+// phi -= e (FP64 correction)
+__global__ void __launch_bounds__(256)
+k_sub_d(double* __restrict__ a, const double* __restrict__ e, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i] -= e[i];
+}
+
 static unsigned grid_for(long n)
 {
     long b = (n + 255) / 256;
@@ -123,6 +132,14 @@ int vec_sub_f(double* a, const float* e, long n, hipStream_t s)
 {
     if (n <= 0) return OK;
     hipLaunchKernelGGL(k_sub_f, dim3(grid_for(n)), dim3(256), 0, s, a, e, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_sub_d(double* a, const double* e, long n, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    hipLaunchKernelGGL(k_sub_d, dim3(grid_for(n)), dim3(256), 0, s, a, e, n);
     GFD_LAUNCH_CHECK();
     return OK;
 }

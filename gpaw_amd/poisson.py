@@ -436,10 +436,11 @@ class FDPoissonSolver(BasePoissonSolver):
 
     def solve_mixed(self, phi, rho, tol=1e-10, levels=None, pre=2, post=2,
                     coarse_sweeps=8, w0=-1.0, wc=-1.0, inner=1,
-                    max_outer=100):
+                    max_outer=100, fp64_inner=False):
         """FP32 V-cycles inside FP64 iterative refinement until
         |L phi - rho| / |rho| <= tol (phi is the initial guess and the
         result).  Returns (refinement steps, final relative residual).
+        fp64_inner: the same cycles in FP64 (isolates what FP32 gains).
 
         levels: depth of the FP32 hierarchy (None: all levels of the FP64
         one).  The coarsest FP32 level gets coarse_sweeps Jacobi sweeps, so
@@ -457,7 +458,8 @@ class FDPoissonSolver(BasePoissonSolver):
         status = _lib.load().gfd_poisson_solve_mixed(
             h, ptr(st.dev[0]), ptr(st.dev[1]), float(tol), int(levels),
             int(pre), int(post), int(coarse_sweeps), float(w0), float(wc),
-            int(inner), int(max_outer), _lib.ctypes.byref(n),
+            int(inner), int(max_outer), int(bool(fp64_inner)),
+            _lib.ctypes.byref(n),
             _lib.ctypes.byref(rel), stream_handle())
         st.download(0)
         if status == _lib.GFD_ECONVERGE:

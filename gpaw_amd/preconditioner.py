@@ -115,3 +115,92 @@ class Preconditioner:
         if not is_device(residuals):
             return d0.cpu().numpy()
         return d0
+
+
+class BatchedPreconditioner:
+    """The FP32 kinetic preconditioner on many independent real grids of
+    uneven shape at once (no counterpart in the reference, which handles
+    one grid descriptor per Preconditioner).
+
+    Each grid gets the hierarchy of Preconditioner: K0 = the caller's
+    kinetic stencil order (nn, scale -1/2) on the grid, nn=1 kinetic
+    operators on its two coarsenings, nn=1 transfers, and its own step
+    0.66666666 / diag(K0) (gpaw/preconditioner.py:17-33).  One call runs
+    the chain of c/fd_preconditioner.cpp:221-345 on every grid in eight
+    launches (gfd_batch_precond32), FP32 work arrays, FP64 residuals and
+    results."""
+
+    def __init__(self, gds, nn=3):
+        self.gds = list(gds)
+        self.levels = []
+        for gd in self.gds:
+            gd1 = gd.coarsen()
+            gd2 = gd1.coarsen()
+            kin = [Laplace(gd, -0.5, nn), Laplace(gd1, -0.5, 1),
+                   Laplace(gd2, -0.5, 1)]
+            rst = [Transformer(gd, gd1, 1), Transformer(gd1, gd2, 1)]
+            itp = [Transformer(gd1, gd, 1), Transformer(gd2, gd1, 1)]
+            self.levels.append((kin, rst, itp))
+        self.steps = np.array([0.66666666 / k[0].get_diagonal_element()
+                               for k, _, _ in self.levels])
+        self._native = None
+        self._res = None
+
+    def _handle(self):
+        if self._native is None:
+            G = len(self.gds)
+            V = _lib.ctypes.c_void_p
+            ops = (V * (3 * G))(*[k[l]._handle().value
+                                  for k, _, _ in self.levels
+                                  for l in range(3)])
+            rst = (V * (2 * G))(*[r[l]._handle().value
+                                  for _, r, _ in self.levels
+                                  for l in range(2)])
+            # level pair l: interpolation from l + 1 back to l
+            itp = (V * (2 * G))(*[i[l]._handle().value
+                                  for _, _, i in self.levels
+                                  for l in range(2)])
+            w = (_lib.ctypes.c_double * 3)(1.0, 1.0, 1.0)
+            h = V()
+            _lib.check(_lib.load().gfd_batch_poisson_new(
+                G, 3, ops, rst, itp, w, _lib.ctypes.byref(h)),
+                'BatchedPreconditioner')
+            self._native = h
+        return self._native
+
+    def __del__(self):
+        try:
+            if self._native is not None:
+                _lib.load().gfd_batch_poisson_free(self._native)
+                self._native = None
+        except Exception:
+            pass
+
+    def __call__(self, residuals, out=None):
+        """residuals: one float64 device array per grid (its shape);
+        returns (or fills ``out`` with) the preconditioned arrays."""
+        if len(residuals) != len(self.gds):
+            raise ValueError('one residual array per grid')
+        for gd, r in zip(self.gds, residuals):
+            if tuple(r.shape) != tuple(gd.n_c) or r.dtype != torch.float64:
+                raise TypeError('residuals: float64 arrays of the grid '
+                                'shapes')
+        flat = torch.cat([r.reshape(-1) for r in residuals])
+        res_out = torch.empty_like(flat)
+        steps = (_lib.ctypes.c_double * len(self.steps))(*self.steps)
+        _lib.check(_lib.load().gfd_batch_precond32(
+            self._handle(), steps, ptr(flat), ptr(res_out),
+            stream_handle()), 'BatchedPreconditioner')
+        if out is None:
+            out = [None] * len(self.gds)
+        i = 0
+        for g, gd in enumerate(self.gds):
+            n = tuple(int(x) for x in gd.n_c)
+            k = int(np.prod(n))
+            v = res_out[i:i + k].reshape(n)
+            if out[g] is None:
+                out[g] = v
+            else:
+                out[g].copy_(v)
+            i += k
+        return out

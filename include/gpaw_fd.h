@@ -186,12 +186,14 @@ int gfd_poisson_set_graph(gfd_poisson* p, int on);
  * coarse_sweeps at the coarsest, weights w0 at level 0 and wc below; -1
  * keeps the FDPoissonSolver value) and phi -= e; stops when
  * |r| / |rho| <= tol_rel.  Returns GFD_ECONVERGE after max_outer steps.
- * Synchronous; *relres = final |r| / |rho|. */
+ * inner_fp64 != 0 runs the same inner cycles (levels, sweeps, weights,
+ * LDS coarse solve) on FP64 correction arrays instead, so that the two
+ * differ in precision only.  Synchronous; *relres = final |r| / |rho|. */
 int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi, const double* rho,
                             double tol_rel, int levels, int pre, int post,
                             int coarse_sweeps, double w0, double wc,
-                            int inner, int max_outer, int* n_outer,
-                            double* relres, void* stream);
+                            int inner, int max_outer, int inner_fp64,
+                            int* n_outer, double* relres, void* stream);
 
 /* --------------------------------------------------------- batched grids
  * Many independent real Poisson problems of uneven shape (no counterpart
@@ -252,6 +254,18 @@ int gfd_batch_poisson_solve_mixed(gfd_batch_poisson* b, double* const* phis,
                                   int pre, int post, int coarse_sweeps,
                                   int max_outer, int use_graph, int* n_outer,
                                   double* relres, void* stream);
+
+/* The FP32 kinetic preconditioner chain of gfd_fd_precond32 (reference:
+ * fd_precond, c/fd_preconditioner.cpp:221-345, one grid per call) on every
+ * grid of a batch at once.  b: a batch of exactly 3 levels built from
+ * each grid's kinetic operators (K0 on the grid, nn=1 on its two
+ * coarsenings) and nn=1 transformers; steps: host array, one
+ * 0.66666666 / diag(K0) per grid (gpaw/preconditioner.py:19); res, out:
+ * FP64 device arrays of every grid's fine values back to back (batch
+ * order), not overlapping.  Eight launches, FP32 work arrays owned by the
+ * batch; asynchronous on stream. */
+int gfd_batch_precond32(gfd_batch_poisson* b, const double* steps,
+                        const double* res, double* out, void* stream);
 
 /* ------------------------------------------------------- FFT Poisson
  * FastPoissonSolver (gpaw/poisson.py:846-1028): the FD operator is

@@ -9,7 +9,7 @@ from gpaw_amd import workloads
 from gpaw_amd.batch_poisson import BatchedPoissonSolver
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpu_util import dev, host, say
-from helpers import relerr, tspec
+from helpers import relerr, spec, tspec
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-13
@@ -113,3 +113,41 @@ def test_mixed_solve_matches_fp64():
             a = a - a.mean()
             r = r - r.mean()
         assert relerr(a, r) < 1e-8
+
+
+def _precond_ref(kin, rst, itp, res, step):
+    return oracle.fd_precond(spec(kin[0]), spec(kin[1]), spec(kin[2]),
+                             tspec(rst[0]), tspec(rst[1]), tspec(itp[0]),
+                             tspec(itp[1]), res, step)
+
+
+def test_batched_preconditioner():
+    """The FP32 chain on five uneven grids in one set of launches, grid by
+    grid against the FP64 oracle (float32 rounding over ~10 passes: bar
+    2e-6, as for the single-grid FP32 chain) and against the single-grid
+    FP32 preconditioner."""
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import (BatchedPreconditioner,
+                                         Preconditioner)
+    rng = np.random.default_rng(23)
+    gds = [GridDescriptor(N, np.array(N) * (0.3 + 0.02 * i), pbc)
+           for i, (N, pbc) in enumerate(SHAPES)]
+    P = BatchedPreconditioner(gds)
+    assert len(set(P.steps)) > 1  # per-grid steps
+    res = [rng.uniform(-1, 1, tuple(gd.n_c)) for gd in gds]
+    say('batched preconditioner', len(gds), 'grids')
+    out = P([dev(r) for r in res])
+    for g, (gd, r, o) in enumerate(zip(gds, res, out)):
+        kin, rst, itp = P.levels[g]
+        ref = _precond_ref(kin, rst, itp, r, P.steps[g])
+        err = relerr(host(o), ref)
+        assert 1e-12 < err < 2e-6, (g, err)
+        one = Preconditioner(gd, Laplace(gd, -0.5, 3), float, fp32=True)
+        assert one.step == P.steps[g]
+        single = host(one(dev(r), None))
+        assert relerr(host(o), single) < 2e-6, g
+    # a second call on other residuals reuses the tables
+    res2 = [2 * r for r in res]
+    out2 = P([dev(r) for r in res2])
+    for g, o in enumerate(out2):
+        assert relerr(host(o), 2 * host(out[g])) < 1e-6

@@ -221,6 +221,29 @@ def test_uneven_48_all_problems():
     say(f'  worst error against the FP64 solves: {worst:.3g}')
 
 
+def test_uneven_48_preconditioner():
+    """The preconditioner half of the 48-problem workload: the batched FP32
+    chain on every grid against the FP64 oracle grid by grid."""
+    from gpaw_amd.preconditioner import BatchedPreconditioner
+    problems = workloads.uneven_48()
+    gds = [g for g, _ in problems]
+    P = BatchedPreconditioner(gds)
+    rng = np.random.default_rng(4807)
+    res = [rng.uniform(-1, 1, tuple(g.n_c)) for g in gds]
+    say('48 uneven problems: batched preconditioner')
+    out = P([dev(r) for r in res])
+    worst = 0.0
+    for g, (r, o) in enumerate(zip(res, out)):
+        kin, rst, itp = P.levels[g]
+        ref = oracle.fd_precond(spec(kin[0]), spec(kin[1]), spec(kin[2]),
+                                tspec(rst[0]), tspec(rst[1]), tspec(itp[0]),
+                                tspec(itp[1]), r, P.steps[g])
+        e = relerr(host(o), ref)
+        worst = max(worst, e)
+        assert e < 2e-6, (g, tuple(gds[g].N_c), e)
+    say(f'  worst error against the FP64 oracle: {worst:.3g}')
+
+
 @pytest.fixture(scope='module')
 def eigen_128():
     import torch

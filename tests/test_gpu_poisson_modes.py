@@ -64,6 +64,44 @@ def test_mixed_precision_reaches_tolerance():
     assert relerr(a, b) < 1e-8
 
 
+@pytest.mark.parametrize('pbc', [True, (True, False, True)])
+def test_refinement_with_fp64_inner_cycles(pbc):
+    """The refinement solver with the same inner cycles in FP64
+    (fp64_inner): it reaches the tolerance, graph replay and eager launches
+    agree bit for bit, and its solution is the FP32-inner one and the
+    plain FP64 V-cycle one to the accuracy both were driven to."""
+    rng = np.random.default_rng(81)
+    gd = GridDescriptor((64, 48, 32), (12.0, 9.0, 6.0), pbc)
+    rho = rand(rng, tuple(gd.n_c))
+    if gd.pbc_c.all():
+        rho -= rho.mean()
+    s = _solver(gd, nn=3, eps=1e-30)
+    drho = dev(rho)
+    kw = dict(levels=4, pre=2, post=2, coarse_sweeps=8, w0=0.8, inner=2)
+    out = {}
+    for fp64 in (False, True):
+        for graph in (True, False):
+            _lib.load().gfd_poisson_set_graph(s._native, int(graph))
+            phi = dev(np.zeros(tuple(gd.n_c)))
+            say('refinement fp64_inner', fp64, 'graph', graph, pbc)
+            n, rel = s.solve_mixed(phi, drho, 1e-10, fp64_inner=fp64, **kw)
+            assert rel <= 1e-10
+            r = dev(np.zeros(tuple(gd.n_c)))
+            n2 = s.operators[0].residual(phi, drho, r, norm=True)
+            assert (n2 / float((rho**2).sum()))**0.5 <= 1.001e-10
+            out[fp64, graph] = (n, host(phi))
+    _lib.load().gfd_poisson_set_graph(s._native, 1)
+    for fp64 in (False, True):
+        assert out[fp64, True][0] == out[fp64, False][0]
+        assert (out[fp64, True][1] == out[fp64, False][1]).all()
+    a, b = out[False, True][1], out[True, True][1]
+    if gd.pbc_c.all():
+        a, b = a - a.mean(), b - b.mean()
+    assert relerr(a, b) < 1e-8
+    # FP64 inner cycles need no more refinement steps than FP32 ones
+    assert out[True, True][0] <= out[False, True][0]
+
+
 @pytest.mark.parametrize('shape,pbc', [((48, 48, 48), True),
                                        ((40, 24, 32), False),
                                        ((64, 48, 32), (True, False, True))])
