@@ -326,6 +326,50 @@ def stage_uneven_48_precond(dev, gds, reps=10):
             'max_diff_vs_single_grid': diff}
 
 
+def subspace_step(gd, kin, vt, phase, psi, hpsi, out):
+    """The subspace diagonalisation of one k-point on the full band set
+    (gpaw/new/pwfd/wave_functions.py:289-368, grid part): H = dv psi^H H psi
+    (FP64 MFMA, every tile), rocSOLVER zheevd, psi' = V psi (dense FP64
+    MFMA rotation).  MFMA fractions count 8 n^2 X flop per product.  Checks:
+    eigenvalues against numpy on the same matrix."""
+    import torch
+    from gpaw_amd.bands import (apply_hamiltonian, subspace_diagonalize,
+                                subspace_matrix)
+    nb = psi.shape[0]
+    X = int(np.prod(psi.shape[1:]))
+    apply_hamiltonian(kin, vt, psi, hpsi, phase)
+    Hm = torch.empty((nb, nb), dtype=torch.complex128, device=psi.device)
+    ms_h, _ = timed(lambda: subspace_matrix(psi, hpsi, gd.dv, Hm), 3, 1)
+    H0 = Hm.clone()
+    eps = torch.empty(nb, dtype=torch.float64, device=psi.device)
+    small = torch.zeros((nb, 1, 1, 128), dtype=torch.complex128,
+                        device=psi.device)
+    small_out = torch.empty_like(small)
+
+    def eig_only():
+        Hm.copy_(H0)
+        subspace_diagonalize(Hm, small, small_out, eps)
+
+    def full():
+        Hm.copy_(H0)
+        subspace_diagonalize(Hm, psi, out, eps)
+
+    ms_eig, _ = timed(eig_only, 2, 1)
+    ms_full, _ = timed(full, 2, 1)
+    ms_rot = max(ms_full - ms_eig, 1e-6)
+    w = np.linalg.eigvalsh(H0.cpu().numpy(), UPLO='L')
+    e = eps.cpu().numpy()
+    flop = 8.0 * nb * nb * X
+    return {'subspace_matrix_ms': round(ms_h, 3),
+            'subspace_matrix_mfma_frac': round(flop / ms_h / 1e9 / 78.6, 4),
+            'zheevd_ms': round(ms_eig, 3),
+            'rotation_ms': round(ms_rot, 3),
+            'rotation_mfma_frac': round(flop / ms_rot / 1e9 / 78.6, 4),
+            'total_ms': round(ms_h + ms_full, 3),
+            'eigenvalue_err_vs_numpy': float(np.abs(e - w).max() /
+                                             np.abs(w).max())}
+
+
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     """One RMM-DIIS iteration per k-point (4 Bloch k-points) on nbands
     complex bands of 128^3: the residuals R = (-1/2 lap(nn=3) + V - eps_n)
@@ -417,6 +461,8 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     got = P(psi[:8], Kpt)
     pre_err = float((got - ref).abs().max() / ref.abs().max())
     del P64, ref, got
+    # H psi in R, the rotated bands in the block step's (free) PR buffer
+    sub = subspace_step(gd, kin, vt, phases[-1], psi, R, rmm._PR)
     X = 128**3
     nk = len(phases)
     herk_flop = nbands * (nbands + 1) / 2 * X * 8
@@ -442,6 +488,7 @@ def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
             'overlap_mfma_frac': round(herk_flop / ms_s / 1e9 / 78.6, 4),
             'orthonormality_err': orth_err,
             'precond_fp32_relerr_vs_fp64': pre_err,
+            'subspace': sub,
             'bands': nbands, 'kpoints': nk, 'grid': [128, 128, 128],
             'note': 'RMM-DIIS (diis_steps=1) with the FP32 preconditioner; '
                     'residuals, step lengths, overlap and Cholesky in '

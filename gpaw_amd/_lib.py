@@ -95,6 +95,7 @@ SIGNATURES = {
     'gfd_h_apply': (_i, [_vp, _vp, _vp, _vp, _l, _dp, _vp]),
     'gfd_band_overlap': (_i, [_vp, _i, _l, _l, _d, _vp, _i, _vp]),
     'gfd_band_matrix': (_i, [_vp, _vp, _i, _l, _l, _d, _vp, _vp]),
+    'gfd_band_subspace_diag': (_i, [_vp, _i, _vp, _l, _l, _vp, _vp, _vp]),
     'gfd_band_cholesky_orth': (_i, [_vp, _i, _vp, _l, _l, _vp, _vp]),
     'gfd_dot': (_i, [_vp, _vp, _l, _dp, _vp]),
     'gfd_add_scalar': (_i, [_vp, _l, _d, _vp]),

@@ -13,6 +13,9 @@ matrices (FP64 MFMA) and Cholesky orthonormalisation.
 - ``orthonormalize``: gpaw/new/pwfd/wave_functions.py:191-246 for the grid
   part (no PAW projections): S -> L = chol(S), C = L^-1, psi <- C psi
   (Matrix.invcholesky, gpaw/core/matrix.py:382-417)
+- ``subspace_diagonalize``: subspace_eigenvalues + canonical_transformation
+  (gpaw/new/pwfd/wave_functions.py:289-368), grid part: eigenvalues and
+  eigenvectors of H[i, j] = <psi_i|H|psi_j>, psi_m <- sum_k v_m[k] psi_k
 
 Arrays are torch complex128 (or float64 for a real H apply) device tensors
 of shape (nbands, n0, n1, n2).
@@ -161,3 +164,27 @@ def orthonormalize(psi, dv, out=None, S=None):
         ptr(S), nb, ptr(psi), _npts(psi), _npts(psi), ptr(out),
         stream_handle()), 'orthonormalize')
     return out, S
+
+
+def subspace_diagonalize(H, psi, out=None, eps=None):
+    """Eigenvalues of the Hermitian subspace matrix H (complex128 device
+    tensor (nbands, nbands), e.g. subspace_matrix(psi, H psi, dv); its lower
+    part is read) and the canonical rotation out_m = sum_k v_m[k] psi_k.
+    On return H[m, k] = v_m[k] (eigenvectors as rows, as the reference's
+    H_nn after subspace_eigenvalues).  Returns (eps, out); eps ascending,
+    float64 device tensor."""
+    _bands(psi, 'psi')
+    nb = psi.shape[0]
+    _square(H, nb)
+    if out is None:
+        out = torch.empty_like(psi)
+    _bands(out, 'out')
+    if out.shape != psi.shape or out.data_ptr() == psi.data_ptr():
+        raise ValueError('out: the shape of psi, not psi itself')
+    if eps is None:
+        eps = torch.empty(nb, dtype=torch.float64, device=psi.device)
+    _bands_vector(eps, nb)
+    _lib.check(_lib.load().gfd_band_subspace_diag(
+        ptr(H), nb, ptr(psi), _npts(psi), _npts(psi), ptr(out), ptr(eps),
+        stream_handle()), 'subspace_diagonalize')
+    return eps, out

@@ -1640,6 +1640,20 @@ extern "C" int gfd_band_cholesky_orth(void* S, int nbands, const void* psi,
                          npts, (double2*)out, ld, (hipStream_t)stream);
 }
 
+extern "C" int gfd_band_subspace_diag(void* H, int nbands, const void* psi,
+                                      long npts, long ld, void* out,
+                                      double* eps, void* stream)
+{
+    if (!H || !psi || !out || !eps || nbands <= 0 || npts <= 0 ||
+        ld < npts || psi == out) {
+        set_error("gfd_band_subspace_diag: bad arguments");
+        return EINVAL_;
+    }
+    return subspace_diag(band_work((hipStream_t)stream), (double2*)H, nbands,
+                         (const double2*)psi, ld, npts, (double2*)out, ld,
+                         eps, (hipStream_t)stream);
+}
+
 // ==================================================================== misc
 
 extern "C" int gfd_dot(const double* a, const double* b, long n,

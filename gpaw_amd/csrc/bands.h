@@ -43,4 +43,11 @@ int band_update(int dtype, int mode, const double* alpha, double scale,
 int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
                   long ldpsi, long X, double2* out, long ldout, hipStream_t s);
 
+// Subspace diagonalisation: H (row-major n x n Hermitian, lower part
+// read) -> eigenvalues eps (device, ascending) and H[m][k] = v_m[k] with
+// H v_m = eps_m v_m; out_m = sum_k v_m[k] psi_k.  Synchronous.
+int subspace_diag(BandWork& w, double2* H, int n, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, double* eps,
+                  hipStream_t s);
+
 }  // namespace gfd

@@ -9,7 +9,9 @@
 //    are reduced in a fixed order (bit-reproducible).
 //  * the Cholesky factor and its inverse come from rocSOLVER (zpotrf,
 //    ztrtri) on the column-major view of the row-major matrix, and the
-//    rotation Psi <- C Psi is k_rot (FP64 MFMA, the lower triangle only).
+//    rotation Psi <- C Psi is k_rot (FP64 MFMA, the lower triangle only);
+//    the subspace eigenvectors come from rocSOLVER zheevd and rotate Psi
+//    through the dense form of k_rot.
 #include "bands.h"
 
 #include <rocblas/rocblas.h>
@@ -160,11 +162,11 @@ __global__ void __launch_bounds__(512) k_gram(GramArgs a)
             }
 }
 
-// Rotation out[i][x] = sum_{j <= i} C[i][j] psi[j][x] with C lower
-// triangular (row-major n x n; entries above the diagonal are ignored) and
-// psi n rows of X points: 128 bands x 128 points per block, K = bands up to
-// the tile's last row, the same MFMA tiling as k_gram (A = C, B = psi
-// without conjugation).
+// Rotation out[i][x] = sum_j C[i][j] psi[j][x] (row-major n x n C, psi n
+// rows of X points): with `lower` C is lower triangular (entries above the
+// diagonal are ignored, K = bands up to the tile's last row), else dense.
+// 128 bands x 128 points per block, the same MFMA tiling as k_gram (A = C,
+// B = psi without conjugation).
 struct RotArgs {
     const double2* C;
     int n;
@@ -174,6 +176,7 @@ struct RotArgs {
     double2* out;
     long ldout;
     int nti;   // band tiles
+    int lower; // C lower triangular (Cholesky), else dense (eigenvectors)
 };
 
 __global__ void __launch_bounds__(512) k_rot(RotArgs a)
@@ -184,7 +187,7 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
     const long xt = L / a.nti;
     const int i0 = bt * GT;
     const long x0 = xt * GT;
-    const int kend = min(a.n, i0 + GT);
+    const int kend = a.lower ? min(a.n, i0 + GT) : a.n;
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int wr = w >> 1, wc = w & 1;
@@ -210,7 +213,7 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
         for (int q = 0; q < 4; q++) {
             const int gi = i0 + cr + 32 * q;
             const int j = ks + ck;
-            vc[q] = (gi < a.n && j < kend && j <= gi)
+            vc[q] = (gi < a.n && j < kend && (j <= gi || !a.lower))
                         ? a.C[(long)gi * a.n + j]
                         : make_double2(0.0, 0.0);
             const int jp = ks + pr + 4 * q;
@@ -549,8 +552,8 @@ static int blas_ok(rocblas_status st, const char* what)
     return OK;
 }
 
-int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
-                  long ldpsi, long X, double2* out, long ldout, hipStream_t s)
+// the work space's rocBLAS handle (created on first use) on stream s
+static int blas_handle(BandWork& w, hipStream_t s, rocblas_handle* out)
 {
     if (!w.blas) {
         rocblas_handle h;
@@ -558,8 +561,81 @@ int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
         w.blas = h;
         GFD_CHECK_HIP(hipMalloc(&w.info, sizeof(int)));
     }
-    rocblas_handle h = (rocblas_handle)w.blas;
-    GFD_TRY(blas_ok(rocblas_set_stream(h, s), "rocblas_set_stream"));
+    *out = (rocblas_handle)w.blas;
+    return blas_ok(rocblas_set_stream(*out, s), "rocblas_set_stream");
+}
+
+static int rotate(const double2* C, int n, int lower, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, hipStream_t s)
+{
+    RotArgs r;
+    r.C = C;
+    r.n = n;
+    r.psi = psi;
+    r.ldpsi = ldpsi;
+    r.X = X;
+    r.out = out;
+    r.ldout = ldout;
+    r.nti = (n + GT - 1) / GT;
+    r.lower = lower;
+    const long nb = (long)r.nti * ((X + GT - 1) / GT);
+    if (nb > 0x7fffffffL) {
+        set_error("band rotation: too many blocks");
+        return ENOTSUP_;
+    }
+    hipLaunchKernelGGL(k_rot, dim3((unsigned)nb), dim3(512), 0, s, r);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+__global__ void __launch_bounds__(256)
+k_conj(double2* __restrict__ a, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        a[i].y = -a[i].y;
+}
+
+int subspace_diag(BandWork& w, double2* H, int n, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, double* eps,
+                  hipStream_t s)
+{
+    rocblas_handle h;
+    GFD_TRY(blas_handle(w, s, &h));
+    // off-diagonal work of the tridiagonal reduction
+    GFD_TRY(w.ensure_partial(sizeof(double) * (size_t)n, s));
+    // the column-major view of row-major H is H^T = conj(H); its upper
+    // triangle is the lower triangle of H, the part the reference reads
+    // (scipy eigh lower=True)
+    GFD_TRY(blas_ok(rocsolver_zheevd(h, rocblas_evect_original,
+                                     rocblas_fill_upper, n,
+                                     (rocblas_double_complex*)H, n, eps,
+                                     (double*)w.partial,
+                                     (rocblas_int*)w.info),
+                    "rocsolver_zheevd"));
+    int info = 0;
+    GFD_CHECK_HIP(hipMemcpyAsync(&info, w.info, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    GFD_CHECK_HIP(hipStreamSynchronize(s));
+    if (info != 0) {
+        set_error("subspace eigensolver did not converge (zheevd info %d)",
+                  info);
+        return ECONVERGE;
+    }
+    // column m of the view holds an eigenvector of conj(H), i.e. conj(v_m)
+    // with H v_m = eps_m v_m: row m of H is conj(v_m); conjugate to v_m
+    const long nn = (long)n * n;
+    const unsigned g = (unsigned)std::min<long>((nn + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_conj, dim3(g), dim3(256), 0, s, H, nn);
+    GFD_LAUNCH_CHECK();
+    return rotate(H, n, 0, psi, ldpsi, X, out, ldout, s);
+}
+
+int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
+                  long ldpsi, long X, double2* out, long ldout, hipStream_t s)
+{
+    rocblas_handle h;
+    GFD_TRY(blas_handle(w, s, &h));
     // column-major view of the row-major lower triangle = upper triangle
     GFD_TRY(blas_ok(rocsolver_zpotrf(h, rocblas_fill_upper, n,
                                      (rocblas_double_complex*)S, n,
@@ -581,23 +657,7 @@ int cholesky_orth(BandWork& w, double2* S, int n, const double2* psi,
                     "rocsolver_ztrtri"));
     // psi_i <- sum_{j <= i} C_ij psi_j (C = L^-1 in the row-major lower
     // triangle of S)
-    RotArgs r;
-    r.C = S;
-    r.n = n;
-    r.psi = psi;
-    r.ldpsi = ldpsi;
-    r.X = X;
-    r.out = out;
-    r.ldout = ldout;
-    r.nti = (n + GT - 1) / GT;
-    const long nb = (long)r.nti * ((X + GT - 1) / GT);
-    if (nb > 0x7fffffffL) {
-        set_error("cholesky_orth: too many blocks");
-        return ENOTSUP_;
-    }
-    hipLaunchKernelGGL(k_rot, dim3((unsigned)nb), dim3(512), 0, s, r);
-    GFD_LAUNCH_CHECK();
-    return OK;
+    return rotate(S, n, 1, psi, ldpsi, X, out, ldout, s);
 }
 
 }  // namespace gfd

@@ -349,6 +349,19 @@ int gfd_band_matrix(const void* psi, const void* hpsi, int nbands, long npts,
 int gfd_band_cholesky_orth(void* S, int nbands, const void* psi, long npts,
                            long ld, void* out, void* stream);
 
+/* Subspace diagonalisation and canonical rotation, grid part
+ * (gpaw/new/pwfd/wave_functions.py:289-368: subspace_eigenvalues, then
+ * canonical_transformation).  H: row-major nbands x nbands Hermitian
+ * matrix H[i][j] = <psi_i|H|psi_j> (gfd_band_matrix; its lower part is
+ * read, as Matrix.eigh reads the lower half, gpaw/core/matrix.py:419-460).
+ * On return eps (device, nbands doubles) holds the eigenvalues in
+ * ascending order, H[m][k] = v_m[k] with H v_m = eps_m v_m -- the content
+ * of the reference's H_nn after its complex_conjugate -- and
+ * out_m = sum_k v_m[k] psi_k (dense FP64 MFMA rotation).  rocSOLVER zheevd;
+ * out must not alias psi.  Synchronous (checks convergence). */
+int gfd_band_subspace_diag(void* H, int nbands, const void* psi, long npts,
+                           long ld, void* out, double* eps, void* stream);
+
 /* ---------------------------------------------------------------- misc */
 
 /* *result = sum_i a_i b_i over n doubles (b == NULL: sum_i a_i); the

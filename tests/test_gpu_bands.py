@@ -10,7 +10,7 @@ import scipy.linalg as sla
 
 import oracle
 from gpaw_amd.bands import (apply_hamiltonian, band_dots, orthonormalize,
-                            overlap, subspace_matrix)
+                            overlap, subspace_diagonalize, subspace_matrix)
 from gpaw_amd.fd_operators import Laplace
 from gpaw_amd.grid_descriptor import GridDescriptor
 from gpu_util import dev, host, say
@@ -119,3 +119,64 @@ def test_h_residual_and_band_dots(dtype):
     apply_hamiltonian(kin, dev(vt), d, out, ph, eps=rq)
     ov = host(band_dots(d, out, dv))
     assert np.abs(ov).max() < 1e-11 * np.abs(e_ref).max()
+
+
+@pytest.mark.parametrize('nb', [96, 200])
+def test_subspace_diagonalize(nb):
+    """Eigenvalues against numpy's eigh of the same (lower-half) matrix,
+    eigenvectors by H v = eps v, the rotated bands against numpy's rotation
+    with the returned vectors, and the subspace matrix of the rotated bands
+    diagonal with the eigenvalues on it (the canonical form)."""
+    import torch
+    rng = np.random.default_rng(900 + nb)
+    gd = GridDescriptor((12, 16, 10), (2.4, 3.2, 2.0))
+    kin = Laplace(gd, -0.5, 3, complex)
+    ph = bloch_phases(rng, gd.pbc_c)
+    vt = dev(rng.uniform(-2, 0, tuple(gd.n_c)))
+    psi0 = psi_of(rng, nb, tuple(gd.n_c))
+    psi, _ = orthonormalize(dev(psi0), gd.dv)
+    hpsi = torch.empty_like(psi)
+    apply_hamiltonian(kin, vt, psi, hpsi, ph)
+    H = subspace_matrix(psi, hpsi, gd.dv)
+    Hh = host(H)
+    low = np.tril(Hh)
+    Hfull = low + np.tril(Hh, -1).conj().T  # the lower half, as read
+    eps_ref = np.linalg.eigvalsh(Hfull)
+    say('subspace diagonalize', nb)
+    eps, out = subspace_diagonalize(H, psi)
+    e = host(eps)
+    assert np.abs(e - eps_ref).max() < 1e-12 * np.abs(eps_ref).max()
+    assert (np.diff(e) >= 0).all()
+    V = host(H)  # rows v_m
+    r = Hfull @ V.T - V.T * e[None, :]
+    assert np.abs(r).max() < 1e-11 * np.abs(eps_ref).max()
+    assert np.abs(V.conj() @ V.T - np.eye(nb)).max() < 1e-12
+    A = host(psi).reshape(nb, -1)
+    ref = (V @ A).reshape(psi.shape)
+    assert relerr(host(out), ref) < 1e-12
+    # canonical form: <out_m|H|out_k> = eps_m delta_mk
+    hout = torch.empty_like(out)
+    apply_hamiltonian(kin, vt, out, hout, ph)
+    H2 = host(subspace_matrix(out, hout, gd.dv))
+    assert np.abs(H2 - np.diag(e)).max() < 1e-10 * np.abs(e).max()
+
+
+def test_dense_rotation_large_k():
+    """Bands beyond one 128-row tile on both sides of the rotation: 300
+    bands (3 tiles, the last partial) over 7000 points (partial point
+    tile), against numpy."""
+    import torch
+    rng = np.random.default_rng(77)
+    nb = 300
+    H0 = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+    H0 = H0 + H0.conj().T
+    psi = psi_of(rng, nb, (7, 10, 100))
+    eps, out = subspace_diagonalize(dev(H0.copy()), dev(psi))
+    w, U = np.linalg.eigh(H0)  # columns: eigenvectors
+    assert np.abs(host(eps) - w).max() < 1e-12 * np.abs(w).max()
+    # v_m = U[:, m] up to a phase: compare phase-invariant products
+    got = host(out).reshape(nb, -1)
+    ref = U.T @ psi.reshape(nb, -1)
+    ov = np.abs(np.einsum('mx,mx->m', got.conj(), ref))
+    nrm = np.linalg.norm(got, axis=1) * np.linalg.norm(ref, axis=1)
+    assert np.abs(ov / nrm - 1).max() < 1e-12

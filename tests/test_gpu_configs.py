@@ -302,6 +302,27 @@ def test_eigen_128_orthonormalize(eigen_128):
     assert err.item() <= 1e-12
 
 
+def test_eigen_128_subspace_diagonalize(eigen_128):
+    """Subspace diagonalisation on the full band set: H = psi^H H psi,
+    eigenvalues against numpy's on the same matrix, three rotated bands
+    against the rotation recomputed with torch from the returned vectors."""
+    import torch
+    from gpaw_amd.bands import (apply_hamiltonian, subspace_diagonalize,
+                                subspace_matrix)
+    gd, ph, vt_h, vt, kin, psi, R, eps = eigen_128
+    say('subspace diagonalisation, 1024 bands')
+    apply_hamiltonian(kin, vt, psi, R, ph)
+    H = subspace_matrix(psi, R, gd.dv)
+    w = np.linalg.eigvalsh(host(H), UPLO='L')
+    e, out = subspace_diagonalize(H, psi, out=R)
+    assert np.abs(host(e) - w).max() < 1e-12 * np.abs(w).max()
+    V = H  # rows: eigenvectors
+    A = psi.reshape(1024, -1)
+    for m in (0, 511, 1023):
+        ref = (V[m:m + 1] @ A).reshape(psi.shape[1:])
+        assert relerr(host(out[m]), host(ref)) < 1e-12, m
+
+
 def test_eigen_128_fp32_preconditioner(eigen_128):
     from gpaw_amd.preconditioner import Preconditioner
     gd, ph, vt_h, vt, kin, psi, R, eps = eigen_128
