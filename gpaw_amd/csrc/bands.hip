@@ -179,15 +179,19 @@ struct RotArgs {
     int lower; // C lower triangular (Cholesky), else dense (eigenvectors)
 };
 
-__global__ void __launch_bounds__(512) k_rot(RotArgs a)
+typedef double RotStage[GK][GP];
+
+// One 128-band x 128-point output tile (band tile bt, point tile xt) of the
+// rotation.  LOWER: K runs to the tile's last row, and an MFMA whose 16
+// rows x 4 k of C lie wholly above the diagonal (all zero) is skipped.
+template <bool LOWER>
+__device__ inline void rot_tile(const RotArgs& a, int bt, long xt,
+                                RotStage& sCr, RotStage& sCi, RotStage& sPr,
+                                RotStage& sPi)
 {
-    __shared__ double sCr[GK][GP], sCi[GK][GP], sPr[GK][GP], sPi[GK][GP];
-    const int L = xcd_remap_rot(blockIdx.x, gridDim.x);
-    const int bt = L % a.nti;
-    const long xt = L / a.nti;
     const int i0 = bt * GT;
     const long x0 = xt * GT;
-    const int kend = a.lower ? min(a.n, i0 + GT) : a.n;
+    const int kend = LOWER ? min(a.n, i0 + GT) : a.n;
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
     const int wr = w >> 1, wc = w & 1;
@@ -213,7 +217,7 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
         for (int q = 0; q < 4; q++) {
             const int gi = i0 + cr + 32 * q;
             const int j = ks + ck;
-            vc[q] = (gi < a.n && j < kend && (j <= gi || !a.lower))
+            vc[q] = (gi < a.n && j < kend && (!LOWER || j <= gi))
                         ? a.C[(long)gi * a.n + j]
                         : make_double2(0.0, 0.0);
             const int jp = ks + pr + 4 * q;
@@ -249,7 +253,10 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
                 bi[c] = sPi[kl][wc * 64 + c * 16 + li];
             }
 #pragma unroll
-            for (int r = 0; r < 2; r++)
+            for (int r = 0; r < 2; r++) {
+                // rows i0 + wr 32 + r 16 .. +15 against k = ks + 4 kk .. +3
+                if (LOWER && ks + 4 * kk > i0 + wr * 32 + r * 16 + 15)
+                    continue;
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
                     accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
@@ -261,6 +268,7 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
                     acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
                         ai[r], br[c], acci[r][c], 0, 0, 0);
                 }
+            }
         }
         __syncthreads();
     }
@@ -276,6 +284,29 @@ __global__ void __launch_bounds__(512) k_rot(RotArgs a)
                     a.out[(long)row * a.ldout + col] =
                         make_double2(accr[r][c][q], acci[r][c][q]);
             }
+}
+
+// Dense: one block per (band tile, point tile).  LOWER: a block computes
+// band tiles p and nti - 1 - p of its point tile one after the other, so
+// that every block runs the same number of K stages (the tiles of a point
+// tile stay in step and share its psi rows in L2).
+template <bool LOWER>
+__global__ void __launch_bounds__(512) k_rot(RotArgs a)
+{
+    __shared__ double sCr[GK][GP], sCi[GK][GP], sPr[GK][GP], sPi[GK][GP];
+    const int L = xcd_remap_rot(blockIdx.x, gridDim.x);
+    if (!LOWER) {
+        rot_tile<false>(a, L % a.nti, L / a.nti, sCr, sCi, sPr, sPi);
+        return;
+    }
+    const int npair = (a.nti + 1) / 2;
+    const int p = L % npair;
+    const long xt = L / npair;
+    rot_tile<true>(a, p, xt, sCr, sCi, sPr, sPi);
+    if (a.nti - 1 - p != p) {
+        __syncthreads();  // the stage arrays are reused
+        rot_tile<true>(a, a.nti - 1 - p, xt, sCr, sCi, sPr, sPi);
+    }
 }
 
 // out[i][j] = alpha (conj?) sum_chunks partial, for every computed tile
@@ -578,12 +609,18 @@ static int rotate(const double2* C, int n, int lower, const double2* psi,
     r.ldout = ldout;
     r.nti = (n + GT - 1) / GT;
     r.lower = lower;
-    const long nb = (long)r.nti * ((X + GT - 1) / GT);
+    const long per = lower ? (r.nti + 1) / 2 : r.nti;
+    const long nb = per * ((X + GT - 1) / GT);
     if (nb > 0x7fffffffL) {
         set_error("band rotation: too many blocks");
         return ENOTSUP_;
     }
-    hipLaunchKernelGGL(k_rot, dim3((unsigned)nb), dim3(512), 0, s, r);
+    if (lower)
+        hipLaunchKernelGGL(k_rot<true>, dim3((unsigned)nb), dim3(512), 0, s,
+                           r);
+    else
+        hipLaunchKernelGGL(k_rot<false>, dim3((unsigned)nb), dim3(512), 0, s,
+                           r);
     GFD_LAUNCH_CHECK();
     return OK;
 }
