@@ -55,7 +55,8 @@ def parse():
                    help='counter runs: 3 headline applies and 3 copies of '
                         'the same 3.62 GB (calibration), no timing line')
     p.add_argument('--eigen-bands', type=int, default=1024)
-    p.add_argument('--stage', choices=['poisson192', 'uneven48', 'eigen'],
+    p.add_argument('--stage', choices=['poisson192', 'uneven48', 'eigen',
+                                       'precond128', 'density96'],
                    help='run only this stage and print its result (for '
                         'profiling)')
     return p.parse_args()
@@ -370,6 +371,59 @@ def subspace_step(gd, kin, vt, phase, psi, hpsi, out):
                                              np.abs(w).max())}
 
 
+def stage_precond_128(dev, nbands=128, reps=5):
+    """The FP32 kinetic preconditioner on one 128-band block of the
+    1024-band workload (128^3 c128, first k-point): reps applications.
+    Traffic model per fine point: complex128 residual read twice and result
+    written once (48 B), complex64 q0 written and read (16 B), coarse
+    levels ~1/7 of the fine work."""
+    import torch
+    from gpaw_amd import workloads
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import Preconditioner
+    gd = workloads.eigen_128_gd()
+    ph = workloads.eigen_128_kpoints()[0]
+    kin = Laplace(gd, -0.5, 3, complex)
+    P = Preconditioner(gd, kin, complex, block=nbands, fp32=True)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1128)
+    R = torch.randn((nbands, 128, 128, 128), dtype=torch.complex128,
+                    device=dev, generator=g)
+    out = torch.empty_like(R)
+
+    class K:
+        phase_cd = ph
+    ms, _ = timed(lambda: P(R, K, out=out), reps, 1)
+    pts = nbands * 128**3
+    gb = pts * 64 * (1 + 1 / 7) / 1e9
+    return {'ms': round(ms, 3), 'bands': nbands,
+            'ms_per_1024_bands': round(ms * 1024 / nbands, 2),
+            'GPts/s': round(pts / ms / 1e6, 1), 'model_GB': round(gb, 2),
+            'hbm_frac_model': round(gb / ms / 8.0, 4)}
+
+
+def stage_density_96(dev, reps=20):
+    """nn=3 restriction 96^3 -> 48^3 and interpolation back on the four
+    densities of the batched-stencil workload."""
+    import torch
+    from gpaw_amd import workloads
+    from gpaw_amd.transformers import Transformer
+    gdd, dens = workloads.density_96()
+    coarse = gdd.coarsen()
+    R = Transformer(gdd, coarse, 3)
+    I = Transformer(coarse, gdd, 3)
+    d = torch.from_numpy(dens).to(dev)
+    c = torch.empty((4, 48, 48, 48), dtype=torch.float64, device=dev)
+    f = torch.empty_like(d)
+    ms_r, _ = timed(lambda: R.apply(d, c), reps, 3)
+    ms_i, _ = timed(lambda: I.apply(c, f), reps, 3)
+    fine_pts = 4 * 96**3
+    return {'restrict_ms': round(ms_r, 4),
+            'restrict_GPts/s_fine': round(fine_pts / ms_r / 1e6, 1),
+            'interpolate_ms': round(ms_i, 4),
+            'interpolate_GPts/s_fine': round(fine_pts / ms_i / 1e6, 1)}
+
+
 def stage_eigen_128(dev, nbands, steps=2, warmup=1, block=128):
     """One RMM-DIIS iteration per k-point (4 Bloch k-points) on nbands
     complex bands of 128^3: the residuals R = (-1/2 lap(nn=3) + V - eps_n)
@@ -568,6 +622,8 @@ def main():
     if args.stage:
         fn = {'poisson192': lambda: stage_poisson_192(dev),
               'uneven48': lambda: stage_uneven_48(dev),
+              'precond128': lambda: stage_precond_128(dev),
+              'density96': lambda: stage_density_96(dev),
               'eigen': lambda: stage_eigen_128(dev, args.eigen_bands,
                                                steps=1, warmup=1)}
         print(json.dumps({args.stage: fn[args.stage]()}, default=float),

@@ -632,6 +632,25 @@ static size_t xfer_scratch_bytes(const gfd_transformer* t, int dtype,
     return t->plan.K == 2 ? 0 : es * xfer_scratch_elems(t->plan, nin) + 16;
 }
 
+// Time the transfer kernel families for this shape once (not under graph
+// capture), into the transformer's output-sized tuning buffer.
+static int xfer_maybe_tune(gfd_transformer* t, int dtype, const BC& bc,
+                           const void* in, long nin, void* work, int in_wide,
+                           hipStream_t s)
+{
+    if (!xfer_needs_tuning(t->plan, dtype, nin, in_wide)) return OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return OK;
+    }
+    if (cs != hipStreamCaptureStatusNone) return OK;
+    const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);
+    GFD_TRY(t->tune_out.ensure(es * t->ngout() * nin, s));
+    return xfer_tune(t->plan, dtype, bc, in, t->tune_out.p, nin, work,
+                     in_wide, s);
+}
+
 // work: caller-owned work space of xfer_scratch_bytes() (graph-captured
 // drivers), or NULL for the transformer's own.
 static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
@@ -643,6 +662,7 @@ static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
         work = t->scratch.p;
     }
     const BC bc = make_bc(t->periodic, t->real ? nullptr : phases);
+    GFD_TRY(xfer_maybe_tune(t, dtype, bc, in, nin, work, ea.in_wide, s));
     return launch_transfer(t->plan, dtype, bc, in, out, nin, work, ea, s);
 }
 
@@ -1222,6 +1242,15 @@ static int pretune_levels(gfd_poisson* p, Levels& lv, hipStream_t s)
         e.c0 = centre_coef(p->ops[l]);
         GFD_TRY(op_pass(p->ops[l], lv.dtype, lv.tmp[l], lv.res[l], 1,
                         nullptr, EPI_PLAIN, e, s, nullptr));
+        if (l < lv.L && l < p->L) {
+            // the transfer kernel choices of this level pair
+            const BC br = make_bc(p->rst[l]->periodic, nullptr);
+            GFD_TRY(xfer_maybe_tune(p->rst[l], lv.dtype, br, lv.res[l], 1,
+                                    p->xscratch.p, 0, s));
+            const BC bi = make_bc(p->itp[l]->periodic, nullptr);
+            GFD_TRY(xfer_maybe_tune(p->itp[l], lv.dtype, bi, lv.phi[l + 1],
+                                    1, p->xscratch.p, 0, s));
+        }
     }
     return OK;
 }
@@ -1744,6 +1773,7 @@ extern "C" int gfd_set_tuning(const char* key, long value)
     }
     if (std::strcmp(key, "autotune") == 0) {
         xr_set_autotune(value != 0);
+        xfer_set_autotune(value != 0);
         return OK;
     }
     if (std::strcmp(key, "xr_nt") == 0) {
@@ -1760,6 +1790,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
     }
     if (std::strcmp(key, "xfer_passes") == 0) {
         xfer_force_passes(value != 0);
+        return OK;
+    }
+    if (std::strcmp(key, "xfer_march_scale") == 0) {
+        xfer_set_march_scale((int)value);
         return OK;
     }
     if (std::strcmp(key, "xfer_march") == 0) {

@@ -91,6 +91,7 @@ struct gfd_transformer {
     int periodic[3] = {0, 0, 0};
     int real = 1;
     DevBuf scratch;
+    DevBuf tune_out;  // output-sized target of the kernel-choice timing
     long ngin() const
     {
         return (long)plan.nin[0] * plan.nin[1] * plan.nin[2];

@@ -45,6 +45,20 @@ void xfer_force_passes(bool on);
 // Use the x-marching fused kernels (every K; default on).  Off: K = 2 by
 // the one-point-per-thread fused kernels, K > 2 by three 1-D passes.
 void xfer_set_march(bool on);
+// Tiles per resident block of the marching kernels, in units of four
+// (default 1: about four).
+void xfer_set_march_scale(int s);
+
+// Timed choice between the x-marching kernel and the other kernels per
+// transfer shape (on by default; off also forgets the choices).
+void xfer_set_autotune(bool on);
+// Whether this shape has no choice yet (and the tuner is on).
+bool xfer_needs_tuning(const XferPlan& p, int dtype, long nin, int in_wide);
+// Time both kernel families with PLAIN passes of `in` into tune_out
+// (output-sized scratch) and record the faster.  Not under graph capture.
+int xfer_tune(const XferPlan& p, int dtype, const BC& bc, const void* in,
+              void* tune_out, long nin, void* scratch, int in_wide,
+              hipStream_t s);
 
 // The x-marching fused transfer (xfer_march.hip); ENOTSUP_ for a shape
 // its tiles cannot take.

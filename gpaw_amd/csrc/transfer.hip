@@ -18,6 +18,10 @@
 // The weights are exact dyadic rationals.
 #include "transfer.h"
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include <type_traits>
 
 namespace gfd {
@@ -520,6 +524,108 @@ static bool g_xfer_march = true;
 void xfer_force_passes(bool on) { g_xfer_force_passes = on; }
 void xfer_set_march(bool on) { g_xfer_march = on; }
 
+// ------------------------------------------------- kernel choice (timed)
+//
+// Per transfer shape (plan, data type, batch, input width) the faster of
+// the x-marching kernel and the one-point / separable-pass kernels, timed
+// once on the call's own input into a scratch output.  Without an entry the
+// marching kernel runs.
+
+struct XKey {
+    int v[12];
+    bool operator<(const XKey& o) const
+    {
+        return std::lexicographical_compare(v, v + 12, o.v, o.v + 12);
+    }
+};
+
+static std::mutex g_xc_mu;
+static std::map<XKey, int> g_xchoice;  // 0 marching, 1 other kernels
+static bool g_xtune = true;
+
+void xfer_set_autotune(bool on)
+{
+    std::lock_guard<std::mutex> lk(g_xc_mu);
+    g_xtune = on;
+    if (!on) g_xchoice.clear();
+}
+
+static XKey xkey(const XferPlan& p, int dtype, long nin, int in_wide)
+{
+    XKey k;
+    k.v[0] = p.K;
+    k.v[1] = p.interpolate;
+    for (int c = 0; c < 3; c++) {
+        k.v[2 + c] = p.nin[c];
+        k.v[5 + c] = p.nout[c];
+    }
+    k.v[8] = dtype;
+    k.v[9] = (int)std::min<long>(nin, 0x7fffffffL);
+    k.v[10] = in_wide;
+    k.v[11] = 0;
+    return k;
+}
+
+static int launch_other(const XferPlan& p, int dtype, const BC& bc,
+                        const void* in, void* out, long nin, void* scratch,
+                        const XEpiArgs& ea, hipStream_t stream);
+
+bool xfer_needs_tuning(const XferPlan& p, int dtype, long nin, int in_wide)
+{
+    std::lock_guard<std::mutex> lk(g_xc_mu);
+    return g_xtune && g_xfer_march && !g_xfer_force_passes &&
+           g_xchoice.find(xkey(p, dtype, nin, in_wide)) == g_xchoice.end();
+}
+
+int xfer_tune(const XferPlan& p, int dtype, const BC& bc, const void* in,
+              void* tune_out, long nin, void* scratch, int in_wide,
+              hipStream_t s)
+{
+    XEpiArgs ea;
+    ea.in_wide = in_wide;
+    hipEvent_t e0, e1;
+    GFD_CHECK_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("transfer tuner: hipEventCreate failed");
+        return EHIP;
+    }
+    float best = 1e30f;
+    int choice = 0;
+    for (int c = 0; c < 2; c++) {
+        auto run = [&]() {
+            return c == 0 ? launch_transfer_march(p, dtype, bc, in, tune_out,
+                                                  nin, ea, s)
+                          : launch_other(p, dtype, bc, in, tune_out, nin,
+                                         scratch, ea, s);
+        };
+        if (run() != OK) {  // this kernel cannot take the shape
+            set_error("");
+            continue;
+        }
+        (void)hipEventRecord(e0, s);
+        int st = OK;
+        for (int r = 0; r < 3 && st == OK; r++) st = run();
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess || st != OK) {
+            (void)hipGetLastError();
+            set_error("");
+            continue;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            choice = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_xc_mu);
+    g_xchoice[xkey(p, dtype, nin, in_wide)] = choice;
+    return OK;
+}
+
 int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
                     const void* in, void* out, long nin, void* scratch,
                     const XEpiArgs& ea, hipStream_t stream)
@@ -528,13 +634,27 @@ int launch_transfer(const XferPlan& p, int dtype, const BC& bc,
         set_error("transfer: FP64 input only for FP32 data and K = 2");
         return EINVAL_;
     }
-    if (g_xfer_march && !g_xfer_force_passes) {
+    bool march = g_xfer_march && !g_xfer_force_passes;
+    if (march) {
+        std::lock_guard<std::mutex> lk(g_xc_mu);
+        auto it = g_xchoice.find(xkey(p, dtype, nin, ea.in_wide));
+        if (it != g_xchoice.end() && it->second == 1) march = false;
+    }
+    if (march) {
         // the x-marching fused kernel (xfer_march.hip), any K
         const int st =
             launch_transfer_march(p, dtype, bc, in, out, nin, ea, stream);
         if (st != ENOTSUP_) return st;
         set_error("");
     }
+    return launch_other(p, dtype, bc, in, out, nin, scratch, ea, stream);
+}
+
+// the one-point-per-thread K = 2 kernels, or the three separable passes
+static int launch_other(const XferPlan& p, int dtype, const BC& bc,
+                        const void* in, void* out, long nin, void* scratch,
+                        const XEpiArgs& ea, hipStream_t stream)
+{
     if (dtype == 3 && p.K != 2) {
         set_error("transfer: complex64 data only for K = 2");
         return ENOTSUP_;

@@ -419,7 +419,9 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * for plain passes on real data and order << 52 its block order (0
  * XCD-contiguous tiles, 1 linear, 2 bands interleaved over the XCDs);
  * "autotune" (0/1, default 1) times candidate full-row tiles on the first
- * launch of each shape (off: heuristic tiles); "xdma" (0/1, default 1)
+ * launch of each shape (off: heuristic tiles) and, per transfer shape, the
+ * x-marching kernel against the other transfer kernels (off: x-marching);
+ * "xdma" (0/1, default 1)
  * lets it time the LDS-DMA variant too; "xr_nt" (-1 automatic, else
  * bit 0 non-temporal stores, bit 1 non-temporal loads);
  * "coarse_lds" (0/1, default 1) runs the coarse levels of the Poisson
@@ -427,9 +429,10 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * "coarse_max_points" (default and 0: 4096) caps the points of its
  * first level;
  * "xfer_passes" (0/1) runs K = 2 transfers as three 1-D passes instead of
- * the fused kernel; "xfer_march" (0/1, default 1) runs every transfer
- * through the x-marching fused kernel (off: one point per thread for
- * K = 2, three 1-D passes for K > 2).
+ * the fused kernel; "xfer_march" (0/1, default 1) lets transfers use the
+ * x-marching fused kernel (off: one point per thread for K = 2, three 1-D
+ * passes for K > 2); "xfer_march_scale" (>= 1, default 1) sets its tiles
+ * per resident block to about 4 x value.
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

@@ -41,11 +41,15 @@ def general_path(on=True):
 
 @contextlib.contextmanager
 def xfer_path(name='march'):
-    """Route transfers through the x-marching kernel ('march', default) or
-    the one-point-per-thread / 1-D pass kernels ('legacy') while active."""
+    """Route transfers through the x-marching kernel ('march', default;
+    the timed kernel choice is off meanwhile, so it is not bypassed) or the
+    one-point-per-thread / 1-D pass kernels ('legacy') while active."""
     lib = _lib.load()
     lib.gfd_set_tuning(b'xfer_march', 1 if name == 'march' else 0)
+    if name == 'march':
+        lib.gfd_set_tuning(b'autotune', 0)
     try:
         yield
     finally:
         lib.gfd_set_tuning(b'xfer_march', 1)
+        lib.gfd_set_tuning(b'autotune', 1)
