@@ -21,7 +21,8 @@
 //    has landed, so every wave issues the same number of DMAs per plane
 //    and the vmcnt bound of each wait is exact.
 //  * synchronisation per plane: wait until this wave's DMA of plane i has
-//    landed (vmcnt = DMAs issued after it), write zero chunks, one barrier,
+//    landed (vmcnt = DMAs issued after it, an immediate per count of
+//    planes in flight: A = 1 .. 4), write zero chunks, one barrier,
 //    then issue the DMA of plane i + A into the slot read at step i - 1
 //    (every wave is past that step), so the ring needs A + 1 slots.
 //    Vector memory operations complete in issue order, so stores issued
@@ -226,11 +227,16 @@ __global__ void __launch_bounds__(XR_MAXT) k_xdma(XdArgs<T> a)
             const int i = ig + u;
             if (i < NL) {
                 // DMAs issued after landing i: planes i+1 .. min(i+A-1,
-                // NL-1), KD each (A is 1 or 2)
-                if (A == 1 || i + 1 >= NL)
+                // NL-1), KD each (A is 1 to 4)
+                const int m = min(A - 1, NL - 1 - i);
+                if (m <= 0)
                     __builtin_amdgcn_s_waitcnt(XD_VMCNT(0) & ~0xF00);
-                else
+                else if (m == 1)
                     __builtin_amdgcn_s_waitcnt(XD_VMCNT(KD) & ~0xF00);
+                else if (m == 2)
+                    __builtin_amdgcn_s_waitcnt(XD_VMCNT(2 * KD) & ~0xF00);
+                else
+                    __builtin_amdgcn_s_waitcnt(XD_VMCNT(3 * KD) & ~0xF00);
                 asm volatile("" ::: "memory");
                 zero_fill(i, s_cur);
                 __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
@@ -348,7 +354,7 @@ inline size_t xd_lds_bytes(const XmTile& t, int R, int es, const int n[3],
                            int* kd_out = nullptr)
 {
     const int PZ = 16 / es;
-    if (t.dma < 1 || t.dma > 2 || n[2] % PZ != 0 || 2 * R > 2 * t.nseg) return 0;
+    if (t.dma < 1 || t.dma > 4 || n[2] % PZ != 0 || 2 * R > 2 * t.nseg) return 0;
     const int CPR = n[2] / PZ;
     const int NT = CPR * t.nseg;
     const int TH = t.nseg * t.py + 2 * R;

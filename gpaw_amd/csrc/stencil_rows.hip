@@ -262,14 +262,15 @@ std::vector<XmTile> xr_candidates(int R, int dtype, const int n[3], bool dma)
             }
     }
     if (dma && (dtype == 0 || dtype == 2)) {
-        // LDS-DMA ring: 1-2 planes in flight, x chunks of 16-64 planes or
+        // LDS-DMA ring: 1-3 planes in flight, x chunks of 16-64 planes or
         // the whole grid (measured on 256 x 96^3 f64 nn=3: tall tiles and
-        // whole-grid chunks win, 8 x 48 threads x 4 rows, 2 planes ahead)
+        // whole-grid chunks win, 8 x 48 threads x 4 rows, 2 planes ahead;
+        // 3 ahead keeps two blocks per CU only for slots up to 20 KB)
         static const int dxcs[] = {16, 32, 1 << 30};
         for (int py = 1; py <= 4; py++)
             for (int sg = 1; sg * TZT <= XR_MAXT; sg++) {
                 if (n[1] % (sg * py) != 0 || sg * TZT < 64) continue;
-                for (int A = 1; A <= 2; A++)
+                for (int A = 1; A <= 3; A++)
                     for (int xc : dxcs)
                         for (int order = 0; order <= 2; order += 2) {
                             if (xc >= n[0] && xc != dxcs[2]) continue;

@@ -415,7 +415,7 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * general kernel); "force_general" (0/1) is force_path 1/0; "xm_tile" and
  * "xr_tile" with value tz | nseg << 12 | py << 24 | xchunk << 32 override
  * the tile of the tiled / full-row x-march kernel (value 0 clears); for
- * "xr_tile", dma << 48 (1-2 planes in flight) selects the LDS-DMA variant
+ * "xr_tile", dma << 48 (1-4 planes in flight) selects the LDS-DMA variant
  * for plain passes on real data and order << 52 its block order (0
  * XCD-contiguous tiles, 1 linear, 2 bands interleaved over the XCDs);
  * "autotune" (0/1, default 1) times candidate full-row tiles on the first

@@ -1,6 +1,6 @@
 """The LDS-DMA row kernel (k_xdma: plain passes on real data) against the
 oracle, on tiles forced through the tile override: every radius, rows per
-thread 1-4, 1-2 planes in flight, x chunks that do and do not divide the
+thread 1-4, 1-4 planes in flight, x chunks that do and do not divide the
 grid, periodic and zero boundaries.  Tolerance 1e-13 relative max-norm."""
 import numpy as np
 import pytest
@@ -21,7 +21,7 @@ def tile_ok(n, R, nseg, py, A):
     PZ = 2
     if n[2] % PZ or n[1] % (nseg * py) or R > n[0] or R > n[1]:
         return False
-    if nseg < R or A > 2:
+    if nseg < R or A > 4:
         return False
     CPR = n[2] // PZ
     NT = CPR * nseg
@@ -64,9 +64,9 @@ def case_grid(R, PY, H):
 
 @pytest.mark.parametrize('R,PY,H', XD_CASES)
 def test_xdma_every_instance(R, PY, H):
-    """Each k_xdma instantiation on a grid and tile it takes, 1 and 2
-    planes in flight, whole and 5-plane x chunks, the boundary kinds in
-    turn (one in four cases all-zero)."""
+    """Each k_xdma instantiation on a grid and tile it takes, 1 to 4
+    planes in flight (where the ring fits LDS), whole and 5-plane x chunks,
+    the boundary kinds in turn (one in four cases all-zero)."""
     lib = _lib.load()
     i = XD_CASES.index((R, PY, H))
     rng = np.random.default_rng(700 + i)
@@ -78,8 +78,10 @@ def test_xdma_every_instance(R, PY, H):
     op = Laplace(gd, -0.5, R)
     a = rand(rng, (3,) + n)
     ref = ref_apply(op, a)
-    for A in (1, 2):
-        assert tile_ok(n, R, nseg, PY, A)
+    for A in (1, 2, 3, 4):
+        if not tile_ok(n, R, nseg, PY, A):
+            assert A > 2
+            continue
         for xc in (n[0], 5):
             say('xdma', (R, PY, H), n, pbc, nseg, A, xc)
             try:
@@ -116,7 +118,7 @@ def test_xdma_laplace(nn):
         a = rand(rng, (3,) + n)
         ref = ref_apply(op, a)
         tiles = [(s, p, A) for s in range(1, 9) for p in (1, 2, 3, 4)
-                 for A in (1, 2) if tile_ok(n, nn, s, p, A)]
+                 for A in (1, 2, 3) if tile_ok(n, nn, s, p, A)]
         assert tiles, (n, nn)
         for nseg, py, A in tiles:
             for xc in (n[0], 5):
