@@ -21,6 +21,7 @@
 #include "handles.h"
 #include "coarse.h"
 #include "gs.h"
+#include "wstencil.h"
 
 namespace gfd {
 
@@ -1790,6 +1791,20 @@ extern "C" int gfd_set_tuning(const char* key, long value)
     }
     if (std::strcmp(key, "xfer_passes") == 0) {
         xfer_force_passes(value != 0);
+        return OK;
+    }
+    if (std::strcmp(key, "wx_march") == 0) {
+        wx_set_enabled(value != 0);
+        return OK;
+    }
+    if (std::strcmp(key, "wx_tile") == 0) {
+        // py | xchunk << 8 (0: automatic)
+        const int py = (int)(value & 0xff), xc = (int)((value >> 8) & 0xffff);
+        if (py != 0 && py != 1 && py != 2 && py != 4) {
+            set_error("gfd_set_tuning: wx_tile rows per thread 1, 2 or 4");
+            return EINVAL_;
+        }
+        wx_set_tile(py, xc);
         return OK;
     }
     if (std::strcmp(key, "xfer_march_scale") == 0) {

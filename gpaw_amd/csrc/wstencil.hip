@@ -18,6 +18,7 @@
 #include "gs.h"
 #include "handles.h"
 #include "stencil_dev.h"
+#include "wstencil.h"
 
 namespace gfd {
 
@@ -193,6 +194,8 @@ struct gfd_woperator {
     int gs_a0 = 0, gs_a1 = 0; // Gauss-Seidel wavefront order (0: unset)
     DevBuf scratch;
     DevBuf partial;
+    bool axis_ok = false;  // every stencil axis-only: `ax` is set
+    WAxis ax;
     long ng() const { return (long)n[0] * n[1] * n[2]; }
     WArgs args(const double* phases) const
     {
@@ -320,6 +323,40 @@ extern "C" int gfd_woperator_new(int nweights, const int* ncoefs,
     }
     op->R = R;
     op->h_off = off;
+    // per-axis form of every stencil for the weighted x-march kernel
+    if (nweights <= WX_NW && R <= MAXR_FAST) {
+        WAxis& ax = op->ax;
+        ax.nw = nweights;
+        ax.R = R;
+        bool ok = true;
+        for (int w = 0; w < nweights && ok; w++) {
+            ax.c0[w] = 0.0;
+            for (int a = 0; a < 3; a++) {
+                ax.hasw[w][a] = 0;
+                for (int k = 0; k < MAXR_FAST; k++)
+                    ax.cp[w][a][k] = ax.cm[w][a][k] = 0.0;
+            }
+            ax.wts[w] = weights[w];
+            for (int p = start[w]; p < start[w + 1] && ok; p++) {
+                const int* o = &off[3 * p];
+                const int nz = (o[0] != 0) + (o[1] != 0) + (o[2] != 0);
+                if (nz == 0) {
+                    ax.c0[w] += coefs[p];
+                } else if (nz == 1) {
+                    const int a = o[0] ? 0 : (o[1] ? 1 : 2);
+                    const int k = o[a];
+                    if (k > 0)
+                        ax.cp[w][a][k - 1] += coefs[p];
+                    else
+                        ax.cm[w][a][-k - 1] += coefs[p];
+                    if (coefs[p] != 0.0) ax.has[a] = ax.hasw[w][a] = 1;
+                } else {
+                    ok = false;
+                }
+            }
+        }
+        op->axis_ok = ok;
+    }
     if (hipMalloc(&op->d_foff, sizeof(long) * total) != hipSuccess ||
         hipMemcpy(op->d_foff, fo.data(), sizeof(long) * total,
                   hipMemcpyHostToDevice) != hipSuccess) {
@@ -366,6 +403,12 @@ extern "C" int gfd_woperator_apply(gfd_woperator* op, const void* in,
     if (nin == 0) return OK;
     hipStream_t s = (hipStream_t)stream;
     const WArgs a = op->args(op->real ? nullptr : phases);
+    if (op->axis_ok) {
+        const int st = launch_wxmarch(op->ax, op->real, op->n, a.bc, in, out,
+                                      nullptr, nin, 0.0, false, s);
+        if (st != ENOTSUP_) return st;
+        set_error("");
+    }
     const unsigned g = wgrid(op->ng() * nin);
     if (op->real)
         hipLaunchKernelGGL(k_wapply<double>, dim3(g), dim3(256), 0, s, a,
@@ -423,10 +466,23 @@ extern "C" int gfd_woperator_relax(gfd_woperator* op, int method, double* f,
     // sweeps alternate f -> tmp -> f; an odd count ends with a copy back
     double* cur = f;
     double* nxt = tmp;
+    bool fast = op->axis_ok;
     for (int k = 0; k < nrelax; k++) {
-        hipLaunchKernelGGL(k_wrelax, dim3(g), dim3(256), 0, s, a,
-                           (const double*)cur, nxt, src, w);
-        GFD_LAUNCH_CHECK();
+        if (fast) {
+            const int st = launch_wxmarch(op->ax, 1, op->n, a.bc, cur, nxt,
+                                          src, 1, w, true, s);
+            if (st == ENOTSUP_) {
+                fast = false;
+                set_error("");
+            } else {
+                GFD_TRY(st);
+            }
+        }
+        if (!fast) {
+            hipLaunchKernelGGL(k_wrelax, dim3(g), dim3(256), 0, s, a,
+                               (const double*)cur, nxt, src, w);
+            GFD_LAUNCH_CHECK();
+        }
         double* t = cur;
         cur = nxt;
         nxt = t;

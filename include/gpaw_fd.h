@@ -432,7 +432,11 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * the fused kernel; "xfer_march" (0/1, default 1) lets transfers use the
  * x-marching fused kernel (off: one point per thread for K = 2, three 1-D
  * passes for K > 2); "xfer_march_scale" (>= 1, default 1) sets its tiles
- * per resident block to about 4 x value.
+ * per resident block to about 4 x value; "wx_march" (0/1, default 1) runs
+ * weighted operators whose stencils are all axis-only (at most 4 of them)
+ * through the weighted x-march kernel (off: one point per thread);
+ * "wx_tile" (py | xchunk << 8, 0 automatic) overrides its rows per thread
+ * (1, 2, 4) and x planes per block.
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 
