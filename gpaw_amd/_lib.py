@@ -68,6 +68,7 @@ SIGNATURES = {
     'gfd_batch_poisson_tune': (_i, [_vp, _vp]),
     'gfd_batch_poisson_tiles': (_i, [_vp, _i, _ip, _ip, _ip,
                                      ctypes.POINTER(_l)]),
+    'gfd_batch_poisson_tile_info': (_i, [_vp, _i, ctypes.POINTER(_l), _i]),
     'gfd_batch_poisson_apply': (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     'gfd_batch_poisson_transfer': (_i, [_vp, _i, _i, _i, _vp, _vp, _vp,
                                         _vp]),

@@ -104,24 +104,33 @@ class BatchedPoissonSolver:
             pass
 
     def tune(self):
-        """Time the candidate tile shapes on this batch (also done on the
-        first solve); returns {level: (bz, by, pz, blocks)}, level -1 being
-        the FP64 fine-level residual."""
+        """Time the candidate tile shapes of both stencil kernels on this
+        batch (also done on the first solve); returns tiles()."""
         lib = _lib.load()
         _lib.check(lib.gfd_batch_poisson_tune(self._handle(),
                                               stream_handle()), 'tune')
         return self.tiles()
 
     def tiles(self):
+        """{level: dict} of the chosen kernel and tile per level, level -1
+        being the FP64 fine-level residual: 'kernel' ('regs' register
+        tiles or 'march' x-march tiles), 'xc' x planes per tile, 'tiles',
+        and bz / by / pz / blocks (register tiles) or nt / py / zmax / pf
+        (x-march tiles, pf planes of loads in flight)."""
         lib = _lib.load()
         out = {}
         c = _lib.ctypes
         for level in range(-1, self.L):
-            bz, by, pz, nb = c.c_int(), c.c_int(), c.c_int(), c.c_long()
-            _lib.check(lib.gfd_batch_poisson_tiles(
-                self._handle(), level, c.byref(bz), c.byref(by), c.byref(pz),
-                c.byref(nb)), 'tiles')
-            out[level] = (bz.value, by.value, pz.value, nb.value)
+            v = (c.c_long * 11)()
+            _lib.check(lib.gfd_batch_poisson_tile_info(self._handle(), level,
+                                                       v, 11), 'tiles')
+            d = {'kernel': 'march' if v[0] == 1 else 'regs', 'xc': v[1],
+                 'tiles': v[2]}
+            if v[0] == 1:
+                d.update(nt=v[7], py=v[8], zmax=v[9], pf=v[10])
+            else:
+                d.update(bz=v[3], by=v[4], pz=v[5], blocks=v[6])
+            out[level] = d
         return out
 
     def level_points(self, level):

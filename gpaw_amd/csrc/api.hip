@@ -22,6 +22,7 @@
 #include "coarse.h"
 #include "gs.h"
 #include "wstencil.h"
+#include "batch.h"
 
 namespace gfd {
 
@@ -1805,6 +1806,14 @@ extern "C" int gfd_set_tuning(const char* key, long value)
             return EINVAL_;
         }
         wx_set_tile(py, xc);
+        return OK;
+    }
+    if (std::strcmp(key, "batch_kernel") == 0) {
+        if (value < 0 || value > 2) {
+            set_error("gfd_set_tuning: batch_kernel 0, 1 or 2");
+            return EINVAL_;
+        }
+        batch_set_kernel((int)value);
         return OK;
     }
     if (std::strcmp(key, "xfer_march_scale") == 0) {

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <vector>
 
+#include "batch.h"
 #include "coarse.h"
 #include "gfd_common.h"
 #include "handles.h"
@@ -28,56 +29,6 @@
 #include "vector.h"
 
 namespace gfd {
-
-constexpr int BMAXR = 3;
-
-// One grid of one level in a batched stencil launch.
-struct BSt {
-    long off;          // element offset in the level arrays
-    long tile0;        // first tile of the grid in the launch
-    int n[3];
-    int periodic[3];
-    int tz, ty;        // tiles along z and y
-    int xc;            // x planes per tile (tiles = ceil(n0 / xc) ty tz)
-    int has[3];
-    double c0, w;      // w: Jacobi weight
-    double step;       // step of EPI_UPDATE / EPI_UPDW
-    double cp[3][BMAXR], cm[3][BMAXR];
-};
-
-struct BSArgs {
-    const BSt* g;
-    int G;
-    long ntiles;
-    const void* in;
-    void* out;
-    const void* aux;
-    double* partial;   // one double per tile (EPI_RESID)
-    int negate;        // EPI_UPDATE / EPI_UPDW: store the negated result
-};
-
-__device__ inline int bfind_tile(const BSt* g, int G, long t)
-{
-    int lo = 0, hi = G - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (g[mid].tile0 <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-// index along an axis with periodic wrap; false if outside a zero boundary
-__device__ inline bool bwrap(int& i, int n, int periodic)
-{
-    if (i >= 0 && i < n) return true;
-    if (!periodic) return false;
-    i %= n;
-    if (i < 0) i += n;
-    return true;
-}
 
 // T: data read, TO: data written (float output of an FP64 residual).
 // EPI_PLAIN, EPI_RELAX (aux = right-hand side), EPI_RESID (aux = rhs,
@@ -423,9 +374,13 @@ __global__ void __launch_bounds__(256) k_bxfer2(BXArgs a)
 // Tile shape of one level: z threads, y threads, z points per thread,
 // resident blocks (0: one block per tile).
 struct BTile {
+    int kind = 0;                   // 0: register tiles, 1: x-march tiles
     int bz = 64, by = 4, pz = 1;
     int xc = 16;                    // x planes per tile (0: whole grid)
     long blocks = 0;
+    // x-march: threads per block, rows per thread, widest z part
+    int nt = 256, py = 4, zmax = 128, pf = 1;   // pf: planes ahead
+    long lds_elems = 0;             // x-march: LDS elements per block
 };
 
 struct BLevel {
@@ -478,18 +433,40 @@ struct gfd_batch_poisson {
 
 namespace gfd {
 
-static void fill_tiles(BLevel& lv, const BTile& t)
+// Tile counts of every grid for tile shape t; false if an x-march tile
+// does not fit (halo per thread, LDS for FP64 data: any level may be
+// applied in FP64).
+static bool fill_tiles(BLevel& lv, const BTile& t)
 {
     lv.tile = t;
     long t0 = 0;
+    long lds = 0;
+    bool ok = true;
+    const int R = lv.R;
     for (BSt& s : lv.st) {
-        s.tz = (s.n[2] + t.bz * t.pz - 1) / (t.bz * t.pz);
-        s.ty = (s.n[1] + t.by - 1) / t.by;
         s.xc = t.xc > 0 ? std::min(t.xc, s.n[0]) : s.n[0];
+        if (t.kind == 1) {
+            // the row in equal parts of at most zmax (and nt) columns
+            const int zw = std::min(t.zmax, t.nt);
+            s.tz = (s.n[2] + zw - 1) / zw;
+            s.mz = (s.n[2] + s.tz - 1) / s.tz;
+            s.mseg = std::max(1, std::min(t.nt / s.mz,
+                                          (s.n[1] + t.py - 1) / t.py));
+            const int TY = s.mseg * t.py;
+            s.ty = (s.n[1] + TY - 1) / TY;
+            ok = ok && 2 * R * (s.mz + TY) <= BXM_HMAX * t.nt;
+            lds = std::max(lds, 2L * (TY + 2 * R) * (s.mz + 2 * R));
+        } else {
+            s.tz = (s.n[2] + t.bz * t.pz - 1) / (t.bz * t.pz);
+            s.ty = (s.n[1] + t.by - 1) / t.by;
+            s.mz = s.mseg = 0;
+        }
         s.tile0 = t0;
         t0 += (long)((s.n[0] + s.xc - 1) / s.xc) * s.ty * s.tz;
     }
     lv.ntiles = t0;
+    lv.tile.lds_elems = lds;
+    return ok && lds * 8 <= 64 * 1024 && t0 <= 0x7fffffffL;
 }
 
 static int upload_level(BLevel& lv, hipStream_t s)
@@ -549,6 +526,21 @@ static int bst(const BLevel& lv, int dtype, int epi, const void* in,
                void* out, const void* aux, double* partial, hipStream_t s,
                int negate = 0)
 {
+    if (lv.tile.kind == 1) {
+        BSArgs a;
+        a.g = (const BSt*)lv.d_st.p;
+        a.G = (int)lv.st.size();
+        a.ntiles = lv.ntiles;
+        a.in = in;
+        a.out = out;
+        a.aux = aux;
+        a.partial = partial;
+        a.negate = negate;
+        // element read: FP32 for dtype 2 (EPI_UPDW included), else FP64
+        const size_t es = dtype == 2 ? 4 : 8;
+        return launch_bxmarch(a, lv.R, lv.tile.py, lv.tile.pf, lv.tile.nt,
+                              lv.tile.lds_elems * es, dtype, epi, s);
+    }
     if (dtype == 0 && epi == EPI_PLAIN)
         return bst_launch_r<double, double, EPI_PLAIN>(lv, in, out, aux,
                                                        partial, s);
@@ -634,58 +626,84 @@ static int bnorms(const BLevel& lv, const double* partial, double* out,
     return OK;
 }
 
-// Candidate tile shapes, timed with a PLAIN pass over the level.
+static int g_bkind = 0;  // 0: timed choice, 1: register tiles, 2: x-march
+
+void batch_set_kernel(int kind) { g_bkind = kind; }
+
+// Candidate tile shapes of both kernels, timed with a PLAIN pass over the
+// level (dtype 4: the FP64 data of the FP64 residual).
 static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
                       const void* in, void* out, hipStream_t s)
 {
-    static const int shapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 2},
-                                    {32, 4}, {16, 8}};
-    static const int pzs[] = {1, 2, 4};
-    static const int xcs[] = {16, 32};
-    const long res[] = {0, (long)b->num_cu * 8};
+    std::vector<BTile> cand;
+    if (g_bkind != 2) {
+        static const int shapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 2},
+                                        {32, 4}, {16, 8}};
+        static const int pzs[] = {1, 2, 4};
+        static const int xcs[] = {16, 32};
+        const long res[] = {0, (long)b->num_cu * 8};
+        for (const auto& sh : shapes)
+            for (int pz : pzs)
+                for (int ci = 0; ci < 4; ci++) {
+                    BTile t;
+                    t.bz = sh[0];
+                    t.by = sh[1];
+                    t.pz = pz;
+                    t.xc = xcs[ci & 1];
+                    t.blocks = res[ci >> 1];
+                    cand.push_back(t);
+                }
+    }
+    if (g_bkind != 1) {
+        for (int nt : {256, 512})
+            for (int py : {1, 2, 4})
+                for (int zmax : {64, 128})
+                    for (int xc : {8, 16, 32})
+                        for (int pf : {1, 2, 3}) {
+                            BTile t;
+                            t.kind = 1;
+                            t.nt = nt;
+                            t.py = py;
+                            t.zmax = zmax;
+                            t.xc = xc;
+                            t.pf = pf;
+                            cand.push_back(t);
+                        }
+    }
     hipEvent_t e0, e1;
     GFD_CHECK_HIP(hipEventCreate(&e0));
     GFD_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     BTile bt;
+    bool found = false;
     int st = OK;
-    for (const auto& sh : shapes) {
-        for (int pz : pzs) {
-            for (int ci = 0; ci < 4; ci++) {
-                BTile t;
-                t.bz = sh[0];
-                t.by = sh[1];
-                t.pz = pz;
-                t.xc = xcs[ci & 1];
-                t.blocks = res[ci >> 1];
-                fill_tiles(lv, t);
-                if ((st = upload_level(lv, s)) != OK) break;
-                const int ep = EPI_PLAIN;
-                const int dt = dtype == 4 ? 0 : dtype;
-                if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) != OK)
-                    break;
-                (void)hipEventRecord(e0, s);
-                for (int r = 0; r < 3; r++)
-                    if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) !=
-                        OK)
-                        break;
-                (void)hipEventRecord(e1, s);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (st == OK && ms < best) {
-                    best = ms;
-                    bt = t;
-                }
-                if (st != OK) break;
-            }
-            if (st != OK) break;
-        }
+    for (const BTile& t : cand) {
+        if (!fill_tiles(lv, t)) continue;
+        if ((st = upload_level(lv, s)) != OK) break;
+        const int ep = EPI_PLAIN;
+        const int dt = dtype == 4 ? 0 : dtype;
+        if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) != OK) break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3 && st == OK; r++)
+            st = bst(lv, dt, ep, in, out, nullptr, nullptr, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
         if (st != OK) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            bt = lv.tile;
+            found = true;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != OK) return st;
+    if (!found) {
+        set_error("batched stencil: no tile shape fits the level");
+        return ENOTSUP_;
+    }
     fill_tiles(lv, bt);
     return upload_level(lv, s);
 }
@@ -1147,10 +1165,27 @@ extern "C" int gfd_batch_poisson_tiles(gfd_batch_poisson* b, int level,
         return EINVAL_;
     }
     const BTile& t = level < 0 ? b->l0_64.tile : b->lev[level].tile;
-    *bz = t.bz;
-    *by = t.by;
-    *pz = t.pz;
-    *blocks = t.blocks;
+    const bool regs = t.kind == 0;
+    *bz = regs ? t.bz : 0;
+    *by = regs ? t.by : 0;
+    *pz = regs ? t.pz : 0;
+    *blocks = regs ? t.blocks : (level < 0 ? b->l0_64 : b->lev[level]).ntiles;
+    return OK;
+}
+
+extern "C" int gfd_batch_poisson_tile_info(gfd_batch_poisson* b, int level,
+                                           long* info, int ninfo)
+{
+    if (!b || level < -1 || level >= b->L || !info || ninfo < 1) {
+        set_error("gfd_batch_poisson_tile_info: bad arguments");
+        return EINVAL_;
+    }
+    const BLevel& lv = level < 0 ? b->l0_64 : b->lev[level];
+    const BTile& t = lv.tile;
+    const long v[] = {t.kind, t.xc,     lv.ntiles, t.bz,   t.by, t.pz,
+                      t.blocks, t.nt, t.py,      t.zmax, t.pf};
+    const int n = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < ninfo; i++) info[i] = i < n ? v[i] : 0;
     return OK;
 }
 

@@ -223,14 +223,26 @@ int gfd_batch_poisson_set_coarse(gfd_batch_poisson* b, const int* nlev,
 /* Points of all grids at one level (-1 for a bad level). */
 long gfd_batch_poisson_level_points(gfd_batch_poisson* b, int level);
 
-/* Time candidate tile shapes per level on this batch and keep the fastest
- * (done by the first apply / transfer / solve if never called). */
+/* Time candidate tile shapes of both batched stencil kernels (register
+ * tiles, x-march tiles) per level on this batch and keep the fastest (done
+ * by the first apply / transfer / solve if never called). */
 int gfd_batch_poisson_tune(gfd_batch_poisson* b, void* stream);
 
-/* Chosen tile of a level (-1: the FP64 fine-level residual): threads along
- * z and y, z points per thread, resident blocks (0: one block per tile). */
+/* Chosen register tile of a level (-1: the FP64 fine-level residual):
+ * threads along z and y, z points per thread, resident blocks (0: one
+ * block per tile); 0, 0, 0 and the tile count when the level runs on
+ * x-march tiles. */
 int gfd_batch_poisson_tiles(gfd_batch_poisson* b, int level, int* bz,
                             int* by, int* pz, long* blocks);
+
+/* Chosen stencil kernel and tile of a level (-1: the FP64 fine-level
+ * residual), ninfo values of: kernel (0 register tiles, 1 x-march tiles),
+ * x planes per tile, tiles, then for register tiles threads along z and y,
+ * z points per thread, resident blocks (0: one block per tile), and for
+ * x-march tiles threads per block, rows per thread, widest z part of a
+ * row, planes of loads in flight; values past those are 0. */
+int gfd_batch_poisson_tile_info(gfd_batch_poisson* b, int level, long* info,
+                                int ninfo);
 
 /* out = L_level in on every grid, float64 (fp32 = 0) or float32 level
  * arrays; one launch. */
@@ -436,7 +448,9 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * weighted operators whose stencils are all axis-only (at most 4 of them)
  * through the weighted x-march kernel (off: one point per thread);
  * "wx_tile" (py | xchunk << 8, 0 automatic) overrides its rows per thread
- * (1, 2, 4) and x planes per block.
+ * (1, 2, 4) and x planes per block; "batch_kernel" (0 timed choice, 1
+ * register tiles, 2 x-march tiles) limits the candidates of the batched
+ * stencil tuner (takes effect at the next tune).
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

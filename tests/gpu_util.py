@@ -53,3 +53,19 @@ def xfer_path(name='march'):
     finally:
         lib.gfd_set_tuning(b'xfer_march', 1)
         lib.gfd_set_tuning(b'autotune', 1)
+
+
+BATCH_KERNELS = {'auto': 0, 'regs': 1, 'march': 2}
+
+
+@contextlib.contextmanager
+def batch_kernel(name='march'):
+    """Limit the batched stencil tuner to one kernel ('regs' register
+    tiles, 'march' x-march tiles) for batches tuned while active."""
+    lib = _lib.load()
+    _lib.check(lib.gfd_set_tuning(b'batch_kernel', BATCH_KERNELS[name]),
+               'batch_kernel')
+    try:
+        yield
+    finally:
+        lib.gfd_set_tuning(b'batch_kernel', 0)

@@ -8,7 +8,7 @@ import oracle
 from gpaw_amd import workloads
 from gpaw_amd.batch_poisson import BatchedPoissonSolver
 from gpaw_amd.grid_descriptor import GridDescriptor
-from gpu_util import dev, host, say
+from gpu_util import batch_kernel, dev, host, say
 from helpers import relerr, spec, tspec
 
 pytestmark = pytest.mark.gpu
@@ -25,9 +25,17 @@ def batch(levels=3):
     return b
 
 
-def test_apply_per_level():
+@pytest.fixture(params=['regs', 'march'])
+def kernel(request):
+    """Batches tuned under either batched stencil kernel."""
+    with batch_kernel(request.param):
+        yield request.param
+
+
+def test_apply_per_level(kernel):
     import torch
     b = batch()
+    say('batched apply tiles', kernel, b.tune())
     rng = np.random.default_rng(21)
     for level in range(b.L):
         ops = [s.operators[level] for s in b.solvers]
@@ -82,7 +90,7 @@ def test_transfers():
             assert relerr(host(got), ref) < TOL
 
 
-def test_mixed_solve_matches_fp64():
+def test_mixed_solve_matches_fp64(kernel):
     from gpaw_amd.poisson import FDPoissonSolver
     problems = workloads.uneven_48(seed=9, nproblems=6)
     gds = [p[0] for p in problems]
@@ -121,7 +129,7 @@ def _precond_ref(kin, rst, itp, res, step):
                              tspec(itp[1]), res, step)
 
 
-def test_batched_preconditioner():
+def test_batched_preconditioner(kernel):
     """The FP32 chain on five uneven grids in one set of launches, grid by
     grid against the FP64 oracle (float32 rounding over ~10 passes: bar
     2e-6, as for the single-grid FP32 chain) and against the single-grid
