@@ -1808,6 +1808,21 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         wx_set_tile(py, xc);
         return OK;
     }
+    if (std::strcmp(key, "batch_tile") == 0) {
+        // nt | py << 12 | zmax << 16 | xc << 28 | pf << 40 (0: timed)
+        const int nt = (int)(value & 0xfff), py = (int)((value >> 12) & 0xf);
+        const int zmax = (int)((value >> 16) & 0xfff);
+        const int xc = (int)((value >> 28) & 0xfff);
+        const int pf = (int)((value >> 40) & 0xf);
+        if (value != 0 &&
+            (nt < 64 || nt > 512 || nt % 64 || (py != 1 && py != 2 && py != 4) ||
+             zmax < 1 || xc < 1 || pf < 1 || pf > 2)) {
+            set_error("gfd_set_tuning: batch_tile fields out of range");
+            return EINVAL_;
+        }
+        batch_force_tile(value ? nt : 0, py, zmax, xc, pf);
+        return OK;
+    }
     if (std::strcmp(key, "batch_kernel") == 0) {
         if (value < 0 || value > 2) {
             set_error("gfd_set_tuning: batch_kernel 0, 1 or 2");

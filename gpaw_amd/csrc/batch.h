@@ -61,13 +61,15 @@ __device__ inline bool bwrap(int& i, int n, int periodic)
 }
 
 // x-march batched stencil (bxmarch.hip): 1-D blocks of `nt` threads, each
-// grid's tile TZ = mz columns by mseg * py rows, pf planes (1 .. 3) of
+// grid's tile TZ = mz columns by mseg * py rows, pf planes (1, 2) of
 // loads in flight; `lds` dynamic bytes.
 // dtype / epilogue pairs as for the register-tile kernel of batch.hip.
-constexpr int BXM_HMAX = 3;     // halo elements per thread per plane
+constexpr int BXM_HMAX = 2;     // halo elements per thread per plane
 constexpr int BXM_MAXT = 512;
 // 0: timed choice of the two kernels, 1: register tiles, 2: x-march
 void batch_set_kernel(int kind);
+// x-march tile used by every level tuned afterwards (nt 0: timed choice)
+void batch_force_tile(int nt, int py, int zmax, int xc, int pf);
 int launch_bxmarch(const BSArgs& a, int R, int py, int pf, int nt,
                    size_t lds, int dtype, int epi, hipStream_t s);
 

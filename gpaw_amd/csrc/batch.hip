@@ -454,8 +454,11 @@ static bool fill_tiles(BLevel& lv, const BTile& t)
                                           (s.n[1] + t.py - 1) / t.py));
             const int TY = s.mseg * t.py;
             s.ty = (s.n[1] + TY - 1) / TY;
-            ok = ok && 2 * R * (s.mz + TY) <= BXM_HMAX * t.nt;
-            lds = std::max(lds, 2L * (TY + 2 * R) * (s.mz + 2 * R));
+            // halo loads: rows, and columns unless the tile is whole rows
+            const int hc = s.tz > 1 ? TY : 0;
+            ok = ok && 2 * R * (s.mz + hc) <= BXM_HMAX * t.nt;
+            // two plane buffers, each with a spare element
+            lds = std::max(lds, 2L * ((TY + 2 * R) * (s.mz + 2 * R) + 1));
         } else {
             s.tz = (s.n[2] + t.bz * t.pz - 1) / (t.bz * t.pz);
             s.ty = (s.n[1] + t.by - 1) / t.by;
@@ -466,6 +469,11 @@ static bool fill_tiles(BLevel& lv, const BTile& t)
     }
     lv.ntiles = t0;
     lv.tile.lds_elems = lds;
+    // x-march buffer resources span one plane: 32-bit byte offsets of
+    // FP64 data below the out-of-range marker 2^31
+    long pmax = 0;
+    for (const BSt& s : lv.st) pmax = std::max(pmax, (long)s.n[1] * s.n[2]);
+    if (t.kind == 1 && pmax * 8 >= 0x7fffffffL) ok = false;
     return ok && lds * 8 <= 64 * 1024 && t0 <= 0x7fffffffL;
 }
 
@@ -627,8 +635,22 @@ static int bnorms(const BLevel& lv, const double* partial, double* out,
 }
 
 static int g_bkind = 0;  // 0: timed choice, 1: register tiles, 2: x-march
+static bool g_bforced = false;
+static BTile g_bforce;   // x-march tile every level must use when forced
 
 void batch_set_kernel(int kind) { g_bkind = kind; }
+
+void batch_force_tile(int nt, int py, int zmax, int xc, int pf)
+{
+    g_bforced = nt > 0;
+    g_bforce = BTile();
+    g_bforce.kind = 1;
+    g_bforce.nt = nt;
+    g_bforce.py = py;
+    g_bforce.zmax = zmax;
+    g_bforce.xc = xc;
+    g_bforce.pf = pf;
+}
 
 // Candidate tile shapes of both kernels, timed with a PLAIN pass over the
 // level (dtype 4: the FP64 data of the FP64 residual).
@@ -636,6 +658,13 @@ static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
                       const void* in, void* out, hipStream_t s)
 {
     std::vector<BTile> cand;
+    if (g_bforced) {
+        if (!fill_tiles(lv, g_bforce)) {
+            set_error("batched stencil: forced x-march tile does not fit");
+            return ENOTSUP_;
+        }
+        return upload_level(lv, s);
+    }
     if (g_bkind != 2) {
         static const int shapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 2},
                                         {32, 4}, {16, 8}};
@@ -659,7 +688,7 @@ static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
             for (int py : {1, 2, 4})
                 for (int zmax : {64, 128})
                     for (int xc : {8, 16, 32})
-                        for (int pf : {1, 2, 3}) {
+                        for (int pf : {1, 2}) {
                             BTile t;
                             t.kind = 1;
                             t.nt = nt;

@@ -6,7 +6,8 @@
 // the table: TZ = mz columns (the row split into equal parts of at most the
 // level's z width, so ragged rows waste little) by mseg * PY rows, laid
 // over a 1-D block of `nt` threads (thread t: column t % mz, row segment
-// t / mz; threads past mseg segments only load halo values).  The block
+// t / mz; threads past mseg segments repeat the last segment's loads and
+// store nothing).  The block
 // marches along an x chunk: each thread keeps a register queue of 2R + 1
 // planes for its PY rows (the x terms), the current plane sits in LDS,
 // double buffered with one barrier per plane, for the y and z terms, and
@@ -17,7 +18,12 @@
 // EPI_PLAIN, EPI_RELAX, EPI_RESID (+ per-tile sum of squares), EPI_UPDATE
 // and EPI_UPDW (FP32 data, FP64 aux and out), with the accumulation order
 // of the register-tile kernel (z, then y, then x terms), so both kernels
-// give the same bits.
+// give the same bits except for the Jacobi update, which multiplies by
+// w / c0 (rounded once per grid) where the other divides by c0.
+//
+// All global traffic goes through buffer instructions (the loop is
+// instruction-issue bound, not memory bound): no 64-bit vector addresses,
+// and masking by out-of-range offsets instead of branches and selects.
 #include <type_traits>
 
 #include "batch.h"
@@ -25,6 +31,46 @@
 #include "stencil_dev.h"
 
 namespace gfd {
+
+// Buffer resources: raw loads and stores through a resource whose
+// base is the start of a plane and whose size is the plane's bytes (0 for
+// a zero plane); a byte offset at or past the size reads 0 and drops a
+// store, which is how zero-boundary neighbours, points outside the grid
+// and lanes without an output are masked without branches.
+constexpr unsigned BXM_OOB = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t brsrc(const void* base,
+                                               unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base),
+                                             (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+template <typename T>
+__device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+    } else {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+    }
+}
+
+template <typename T>
+__device__ inline void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __builtin_bit_cast(unsigned, v), r, (int)voff, 0, 0);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r,
+                                              (int)voff, 0, 0);
+    }
+}
 
 template <typename T, typename TO, int R, int PY, int PF, int EPI>
 __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
@@ -46,8 +92,12 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
 
     const int NT = blockDim.x, tid = threadIdx.x;
     const int TZ = d.mz, NSEG = d.mseg, TY = NSEG * PY;
-    const int tx = tid % TZ, sg = tid / TZ;
-    const bool lane_on = sg < NSEG;
+    const int tx = tid % TZ;
+    // threads past the last full segment repeat the last segment's work
+    // (same loads and LDS values) and store nothing
+    const int sg0 = tid / TZ;
+    const bool lane_on = sg0 < NSEG;
+    const int sg = lane_on ? sg0 : NSEG - 1;
     const int LW = TZ + 2 * R;
     const int n0 = d.n[0], n1 = d.n[1], n2 = d.n[2];
     const long plane = (long)n1 * n2;
@@ -55,45 +105,46 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
     const int xs = xt * d.xc, xe = min(xs + d.xc, n0);
     const T* __restrict__ gin = (const T*)a.in + d.off;
     const TA* __restrict__ gaux = (const TA*)a.aux + d.off;
+    TO* __restrict__ gout = (TO*)a.out + d.off;
     const bool has_lds = d.has[1] || d.has[2];
-
-    // Every load below is unconditional, from an address inside the grid
-    // (zero-boundary neighbours read a clamped point and are masked when
-    // used), so the loads of later planes stay in flight while a plane is
-    // computed: the waits the compiler inserts count only older loads.
-    auto clampi = [](int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
+    const unsigned pbytes = (unsigned)(plane * (long)sizeof(T));
 
     // own column and rows: wrapped (periodic) or zero beyond the grid, so
     // the LDS plane of a ragged tile holds what the neighbours must read.
-    // rowoff: in-plane offset (clamped); bit j of vmask: row j is nonzero,
-    // of act: row j is an output point.
+    // off_own: byte offset in the plane (BXM_OOB: zero); off_out: output
+    // offset in elements (BXM_OOB: no output)
     const int zc = z0 + tx;
     int zw = zc;
     const bool okz = bwrap(zw, n2, d.periodic[2]);
-    zw = clampi(zw, n2);
-    int rowoff[PY];
-    unsigned vmask = 0, act = 0;
+    unsigned off_own[PY], off_out[PY];
 #pragma unroll
     for (int j = 0; j < PY; j++) {
         const int yr = y0 + sg * PY + j;
         int yw = yr;
         const bool oky = bwrap(yw, n1, d.periodic[1]);
-        rowoff[j] = clampi(yw, n1) * n2 + zw;
-        if (lane_on && oky && okz) vmask |= 1u << j;
-        if (lane_on && yr < n1 && zc < n2) act |= 1u << j;
+        off_own[j] = (oky && okz) ? (unsigned)(yw * n2 + zw) * sizeof(T)
+                                  : BXM_OOB;
+        off_out[j] = (lane_on && yr < n1 && zc < n2)
+                         ? (unsigned)(yr * n2 + zc) : BXM_OOB;
     }
 
     // halo elements of this thread (rows above / below, columns left /
-    // right of the tile): LDS index (-1: none), clamped offset, bit m of
-    // hmask: nonzero
-    const int H = has_lds ? 2 * R * (TZ + TY) : 0;
-    int h_lds[BXM_HMAX], h_off[BXM_HMAX];
-    unsigned hmask = 0;
+    // right of the tile): LDS index (unused ones write a spare element
+    // past the plane) and byte offset (BXM_OOB: zero or unused).  A tile
+    // of whole rows (one z part) loads no halo columns: the edge threads
+    // copy their value to the wrapped column in LDS (periodic z) or the
+    // columns stay zero.
+    const bool fullrow = d.tz == 1;
+    const bool mirror = fullrow && d.periodic[2] && d.has[2];
+    const int H = has_lds ? 2 * R * TZ + (fullrow ? 0 : 2 * R * TY) : 0;
+    const int LSZ = (TY + 2 * R) * LW + 1;   // + the spare element
+    int h_lds[BXM_HMAX];
+    unsigned h_off[BXM_HMAX];
 #pragma unroll
     for (int m = 0; m < BXM_HMAX; m++) {
         const int h = tid + m * NT;
-        h_lds[m] = -1;
-        h_off[m] = 0;
+        h_lds[m] = LSZ - 1;
+        h_off[m] = BXM_OOB;
         if (h < H) {
             int row, col;
             if (h < 2 * R * TZ) {
@@ -110,42 +161,37 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
             int yw = y0 + row, zw2 = z0 + col;
             const bool okh = bwrap(zw2, n2, d.periodic[2]);
             const bool okv = bwrap(yw, n1, d.periodic[1]);
-            h_off[m] = clampi(yw, n1) * n2 + clampi(zw2, n2);
-            if (okh && okv) hmask |= 1u << m;
+            if (okh && okv) h_off[m] = (unsigned)(yw * n2 + zw2) * sizeof(T);
         }
     }
 
-    // plane x: address of a plane inside the grid, and whether it is
-    // nonzero (inside, or wrapped on a periodic axis)
-    auto pbase = [&](int x) -> long {
+    // resource of input plane x (wrapped on a periodic axis, empty for a
+    // zero plane)
+    auto in_plane = [&](int x) {
         int xw = x % n0;
         if (xw < 0) xw += n0;
-        return (long)xw * plane;
-    };
-    auto pok = [&](int x) {
-        return (x >= 0 && x < n0) || d.periodic[0] != 0;
+        const bool ok = (x >= 0 && x < n0) || d.periodic[0] != 0;
+        return brsrc(gin + (long)xw * plane, ok ? pbytes : 0u);
     };
     auto ld_own = [&](int x, T* dst) {
-        const T* pl = gin + pbase(x);
+        const __amdgpu_buffer_rsrc_t r = in_plane(x);
 #pragma unroll
-        for (int j = 0; j < PY; j++) dst[j] = pl[rowoff[j]];
+        for (int j = 0; j < PY; j++) dst[j] = bload<T>(r, off_own[j]);
     };
     auto ld_halo = [&](int x, T* dst) {
-        const T* pl = gin + pbase(x);
+        const __amdgpu_buffer_rsrc_t r = in_plane(x);
 #pragma unroll
-        for (int m = 0; m < BXM_HMAX; m++) dst[m] = pl[h_off[m]];
+        for (int m = 0; m < BXM_HMAX; m++) dst[m] = bload<T>(r, h_off[m]);
     };
-    // output rows of plane x (x inside the chunk): rowoff is exact there
+    // right-hand side of the output rows of plane x (inside the chunk)
     auto ld_aux = [&](int x, TA* dst) {
-        const TA* pl = gaux + (long)x * plane;
-#pragma unroll
-        for (int j = 0; j < PY; j++) dst[j] = pl[rowoff[j]];
-    };
-    auto own_mask = [&](int x, T* v) {
-        const bool okx = pok(x);
+        const __amdgpu_buffer_rsrc_t r = brsrc(
+            gaux + (long)x * plane, (unsigned)(plane * (long)sizeof(TA)));
 #pragma unroll
         for (int j = 0; j < PY; j++)
-            if (!(okx && ((vmask >> j) & 1u))) v[j] = zero_of<T>();
+            dst[j] = bload<TA>(r, off_out[j] == BXM_OOB
+                                      ? BXM_OOB
+                                      : off_out[j] * (unsigned)sizeof(TA));
     };
 
     T cpx[R], cmx[R], cpy[R], cmy[R], cpz[R], cmz[R];
@@ -158,13 +204,15 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
         cpz[k] = (T)d.cp[2][k];
         cmz[k] = (T)d.cm[2][k];
     }
-    const T c0 = (T)d.c0, w = (T)d.w, step = (T)d.step;
+    const T c0 = (T)d.c0, step = (T)d.step;
+    // Jacobi: (1 - w) v + (w / c0) (f - acc)
+    const T w1 = (T)(1.0 - d.w), wc = (T)(d.w / d.c0);
 
     // q: own values of planes x - R .. x + R.  In flight, slot u of plane
     // x = x0 + u: own values of plane x + R + 1 (qp), halo values of plane
     // x (hp), right-hand side of plane x (ap); each slot is reloaded for
     // plane x + PF as soon as it is consumed (clamped to the chunk at its
-    // end).  The x loop is unrolled by PF, so the slots are renamed
+    // end).  The x loop is unrolled by PF so the buffers are renamed
     // statically.
     T q[2 * R + 1][PY];
     T qp[PF][PY];
@@ -179,28 +227,42 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
         ld_own(min(x + R + 1, xe + R - 1), qp[u]);
         if constexpr (AUX) ld_aux(x, ap[u]);
     }
-#pragma unroll
-    for (int k = 0; k < 2 * R + 1; k++) own_mask(xs - R + k, q[k]);
 
-    TO* __restrict__ out = (TO*)a.out;
     double nrm = 0.0;
     const int lcol = tx + R;
-    const int LSZ = (TY + 2 * R) * LW;
+    if (fullrow && !mirror && tx < R) {
+        // zero halo columns of the own rows, both buffers (never written
+        // again; the first barrier of the x loop orders these stores)
+#pragma unroll
+        for (int j = 0; j < PY; j++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                T* row = lds + b * LSZ + (sg * PY + j + R) * LW;
+                row[tx] = zero_of<T>();
+                row[TZ + R + tx] = zero_of<T>();
+            }
+    }
 
     auto body = [&](const int x, T* qpu, T* hpu, TA* apu) {
         T* buf = lds + (x & 1) * LSZ;
         if (has_lds) {
-            if (lane_on) {
+#pragma unroll
+            for (int j = 0; j < PY; j++)
+                buf[(sg * PY + j + R) * LW + lcol] = q[R][j];
+            // column tx is also column tx + n2 (left edge) and / or
+            // tx - n2 (right edge) of the periodic row
+            if (mirror && tx < R) {
 #pragma unroll
                 for (int j = 0; j < PY; j++)
-                    buf[(sg * PY + j + R) * LW + lcol] = q[R][j];
+                    buf[(sg * PY + j + R) * LW + lcol + TZ] = q[R][j];
             }
-            const bool okx = pok(x);
+            if (mirror && tx >= TZ - R) {
 #pragma unroll
-            for (int m = 0; m < BXM_HMAX; m++)
-                if (h_lds[m] >= 0)
-                    buf[h_lds[m]] =
-                        (okx && ((hmask >> m) & 1u)) ? hpu[m] : zero_of<T>();
+                for (int j = 0; j < PY; j++)
+                    buf[(sg * PY + j + R) * LW + lcol - TZ] = q[R][j];
+            }
+#pragma unroll
+            for (int m = 0; m < BXM_HMAX; m++) buf[h_lds[m]] = hpu[m];
             ld_halo(min(x + PF, xe - 1), hpu);
             __syncthreads();
         }
@@ -210,56 +272,55 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
             for (int j = 0; j < PY; j++) f[j] = apu[j];
             ld_aux(min(x + PF, xe - 1), apu);
         }
-        if (lane_on) {
-            const long obase = d.off + (long)x * plane;
+        const __amdgpu_buffer_rsrc_t ro = brsrc(
+            gout + (long)x * plane, (unsigned)(plane * (long)sizeof(TO)));
 #pragma unroll
-            for (int j = 0; j < PY; j++) {
-                const int lrow = sg * PY + j + R;
-                T acc = zero_of<T>();
-                if (d.has[2]) {
+        for (int j = 0; j < PY; j++) {
+            const int lrow = sg * PY + j + R;
+            T acc = zero_of<T>();
+            if (d.has[2]) {
 #pragma unroll
-                    for (int k = 1; k <= R; k++) {
-                        acc = fmac(acc, cpz[k - 1], buf[lrow * LW + lcol + k]);
-                        acc = fmac(acc, cmz[k - 1], buf[lrow * LW + lcol - k]);
-                    }
+                for (int k = 1; k <= R; k++) {
+                    acc = fmac(acc, cpz[k - 1], buf[lrow * LW + lcol + k]);
+                    acc = fmac(acc, cmz[k - 1], buf[lrow * LW + lcol - k]);
                 }
-                if (d.has[1]) {
-#pragma unroll
-                    for (int k = 1; k <= R; k++) {
-                        const T up = (j + k < PY) ? q[R][(j + k) % PY]
-                                                  : buf[(lrow + k) * LW + lcol];
-                        const T dn = (j - k >= 0)
-                                         ? q[R][(j - k + PY) % PY]
-                                         : buf[(lrow - k) * LW + lcol];
-                        acc = fmac(acc, cpy[k - 1], up);
-                        acc = fmac(acc, cmy[k - 1], dn);
-                    }
-                }
-                if (d.has[0]) {
-#pragma unroll
-                    for (int k = 1; k <= R; k++) {
-                        acc = fmac(acc, cpx[k - 1], q[R + k][j]);
-                        acc = fmac(acc, cmx[k - 1], q[R - k][j]);
-                    }
-                }
-                if (!((act >> j) & 1u)) continue;
-                const long idx = obase + rowoff[j];
-                const T v = q[R][j];
-                T r;
-                if constexpr (EPI == EPI_PLAIN) {
-                    r = fmac(acc, c0, v);
-                } else if constexpr (EPI == EPI_RELAX) {
-                    r = ((T)1.0 - w) * v + w * (f[j] - acc) / c0;
-                } else if constexpr (EPI == EPI_UPDATE || EPI == EPI_UPDW) {
-                    const T qv = fmac(acc, c0, v) - (T)f[j];
-                    r = fmac(v, -step, qv);
-                    if (a.negate) r = -r;
-                } else {
-                    r = fmac(acc, c0, v) - f[j];
-                    nrm += (double)r * (double)r;
-                }
-                out[idx] = (TO)r;
             }
+            if (d.has[1]) {
+#pragma unroll
+                for (int k = 1; k <= R; k++) {
+                    const T up = (j + k < PY) ? q[R][(j + k) % PY]
+                                              : buf[(lrow + k) * LW + lcol];
+                    const T dn = (j - k >= 0) ? q[R][(j - k + PY) % PY]
+                                              : buf[(lrow - k) * LW + lcol];
+                    acc = fmac(acc, cpy[k - 1], up);
+                    acc = fmac(acc, cmy[k - 1], dn);
+                }
+            }
+            if (d.has[0]) {
+#pragma unroll
+                for (int k = 1; k <= R; k++) {
+                    acc = fmac(acc, cpx[k - 1], q[R + k][j]);
+                    acc = fmac(acc, cmx[k - 1], q[R - k][j]);
+                }
+            }
+            const T v = q[R][j];
+            T r;
+            if constexpr (EPI == EPI_PLAIN) {
+                r = fmac(acc, c0, v);
+            } else if constexpr (EPI == EPI_RELAX) {
+                r = fmac(w1 * v, wc, f[j] - acc);
+            } else if constexpr (EPI == EPI_UPDATE || EPI == EPI_UPDW) {
+                const T qv = fmac(acc, c0, v) - (T)f[j];
+                r = fmac(v, -step, qv);
+                if (a.negate) r = -r;
+            } else {
+                r = fmac(acc, c0, v) - f[j];
+                if (off_out[j] != BXM_OOB) nrm += (double)r * (double)r;
+            }
+            bstore<TO>((TO)r, ro,
+                       off_out[j] == BXM_OOB
+                           ? BXM_OOB
+                           : off_out[j] * (unsigned)sizeof(TO));
         }
 #pragma unroll
         for (int k = 0; k < 2 * R; k++)
@@ -267,12 +328,10 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
             for (int j = 0; j < PY; j++) q[k][j] = q[k + 1][j];
 #pragma unroll
         for (int j = 0; j < PY; j++) q[2 * R][j] = qpu[j];
-        own_mask(x + R + 1, q[2 * R]);
         ld_own(min(x + R + 1 + PF, xe + R - 1), qpu);
     };
 
-    // whole groups of PF planes, then the rest (no branch inside a group,
-    // so the waits stay exact in the main loop)
+    // whole groups of PF planes, then the rest (no branch inside a group)
     const int xm = xs + (xe - xs) / PF * PF;
     for (int x0 = xs; x0 < xm; x0 += PF) {
 #pragma unroll
@@ -314,9 +373,9 @@ static int bxm_launch_p(const BSArgs& a, int py, int pf, int nt, size_t lds,
 #define GFD_BXM(PP, FF)                                                        \
     if (py == PP && pf == FF)                                                  \
         return bxm_launch_t<T, TO, R, PP, FF, EPI>(a, nt, lds, s);
-    GFD_BXM(1, 1) GFD_BXM(1, 2) GFD_BXM(1, 3)
-    GFD_BXM(2, 1) GFD_BXM(2, 2) GFD_BXM(2, 3)
-    GFD_BXM(4, 1) GFD_BXM(4, 2) GFD_BXM(4, 3)
+    GFD_BXM(1, 1) GFD_BXM(1, 2)
+    GFD_BXM(2, 1) GFD_BXM(2, 2)
+    GFD_BXM(4, 1) GFD_BXM(4, 2)
 #undef GFD_BXM
     set_error("batched x-march: %d rows / %d planes ahead not built", py, pf);
     return ENOTSUP_;

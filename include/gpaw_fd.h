@@ -450,7 +450,11 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * "wx_tile" (py | xchunk << 8, 0 automatic) overrides its rows per thread
  * (1, 2, 4) and x planes per block; "batch_kernel" (0 timed choice, 1
  * register tiles, 2 x-march tiles) limits the candidates of the batched
- * stencil tuner (takes effect at the next tune).
+ * stencil tuner (takes effect at the next tune); "batch_tile" (nt | py <<
+ * 12 | zmax << 16 | xc << 28 | pf << 40, 0 timed choice) makes every level
+ * tuned afterwards use that x-march tile: nt threads per block (64 .. 512,
+ * multiple of 64), py rows per thread (1, 2, 4), z parts of at most zmax
+ * columns, xc x planes per tile, pf planes of loads in flight (1, 2).
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

@@ -35,7 +35,35 @@ def kernel(request):
 def test_apply_per_level(kernel):
     import torch
     b = batch()
-    say('batched apply tiles', kernel, b.tune())
+    tiles = b.tune()
+    say('batched apply tiles', kernel, tiles)
+    assert all(t['kernel'] == kernel for t in tiles.values())
+    _check_apply(b)
+
+
+@pytest.mark.parametrize('nt,py,zmax,xc,pf', [
+    (256, 1, 64, 8, 1), (256, 2, 32, 5, 2), (512, 4, 128, 16, 1),
+    (512, 4, 128, 7, 2), (128, 2, 16, 3, 2), (64, 1, 16, 40, 1)])
+def test_apply_forced_tiles(nt, py, zmax, xc, pf):
+    """Every x-march variant (rows per thread, loads in flight, z parts
+    narrower than a row, ragged x chunks) on the same five grids."""
+    from gpaw_amd import _lib
+    lib = _lib.load()
+    code = nt | py << 12 | zmax << 16 | xc << 28 | pf << 40
+    _lib.check(lib.gfd_set_tuning(b'batch_tile', code), 'batch_tile')
+    try:
+        b = batch()
+        tiles = b.tune()
+    finally:
+        lib.gfd_set_tuning(b'batch_tile', 0)
+    say('forced tiles', tiles)
+    assert all(t['kernel'] == 'march' and t['py'] == py and t['pf'] == pf
+               for t in tiles.values())
+    _check_apply(b)
+
+
+def _check_apply(b):
+    import torch
     rng = np.random.default_rng(21)
     for level in range(b.L):
         ops = [s.operators[level] for s in b.solvers]
