@@ -1062,8 +1062,11 @@ static double sched_w(const Schedule& sc, int l)
 // first lc >= 1 whose level arrays fit one workgroup's LDS (rhs of level lc
 // stays in global memory).  *changed reports a new plan (graphs that
 // captured the old one must be dropped).  Not callable under capture.
+// cheb: Chebyshev iteration on the coarsest level (mixed schedules; the
+// reference schedule smooths every level with Jacobi)
 static int plan_coarse(gfd_poisson* p, const Schedule& sc, int dtype,
-                       CoarseSolve& c, hipStream_t s, bool* changed)
+                       CoarseSolve& c, hipStream_t s, bool* changed,
+                       bool cheb = false)
 {
     const size_t es = dtype == 2 ? 4 : 8;
     int lc = -1;
@@ -1132,6 +1135,20 @@ static int plan_coarse(gfd_poisson* p, const Schedule& sc, int dtype,
                     L.ioff[d] = p->itp[l]->plan.ax[d].off;
                     L.iskip0[d] = p->itp[l]->plan.ax[d].skip0;
                 }
+        }
+        CLevel& C = g.lev[g.nlev - 1];
+        const int nc = C.n[0] * C.n[1] * C.n[2];
+        double lo, hi;
+        C.cheb = 0;
+        C.dbuf = -1;
+        if (cheb && coarse_cheb_enabled() &&
+            bytes + es * nc <= COARSE_LDS_MAX && cheb_bounds(C, &lo, &hi)) {
+            C.cheb = 1;
+            C.dbuf = off;
+            C.lam_lo = lo;
+            C.lam_hi = hi;
+            off += nc;
+            bytes += es * nc;
         }
         g.lds_elems = off;
     }
@@ -1477,7 +1494,7 @@ extern "C" int gfd_poisson_solve_mixed(gfd_poisson* p, double* phi,
     sc.w0 = w0;
     sc.wc = wc;
     bool cs_changed = false;
-    GFD_TRY(plan_coarse(p, sc, fp64 ? 0 : 2, csi, s, &cs_changed));
+    GFD_TRY(plan_coarse(p, sc, fp64 ? 0 : 2, csi, s, &cs_changed, true));
     const int key[6] = {L32, pre, post, coarse_sweeps, inner, 0};
     bool same = gi && !cs_changed && gw[0] == w0 && gw[1] == wc;
     for (int i = 0; i < 6; i++) same = same && gkey[i] == key[i];
@@ -1784,6 +1801,10 @@ extern "C" int gfd_set_tuning(const char* key, long value)
     }
     if (std::strcmp(key, "coarse_lds") == 0) {
         g_coarse_lds = value != 0;
+        return OK;
+    }
+    if (std::strcmp(key, "coarse_cheb") == 0) {
+        coarse_set_cheb(value != 0);
         return OK;
     }
     if (std::strcmp(key, "coarse_max_points") == 0) {

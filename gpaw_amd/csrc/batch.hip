@@ -427,7 +427,7 @@ struct gfd_batch_poisson {
     std::vector<double> cw;
     std::vector<CGrid> cgrids;
     size_t clds = 0;
-    int ckey[3] = {-1, -1, -1};
+    int ckey[4] = {-1, -1, -1, -1};
     DevBuf d_cgrids;
 };
 
@@ -1067,8 +1067,8 @@ static int b_plan_coarse(gfd_batch_poisson* b, int pre, int post, int coarse,
 {
     *changed = false;
     if (!b->coarse) return OK;
-    const int key[3] = {pre, post, coarse};
-    if (std::equal(key, key + 3, b->ckey)) return OK;
+    const int key[4] = {pre, post, coarse, coarse_cheb_enabled() ? 1 : 0};
+    if (std::equal(key, key + 4, b->ckey)) return OK;
     const int lc = b->L - 1;
     b->cgrids.assign(b->G, CGrid{});
     size_t maxb = 0;
@@ -1122,6 +1122,23 @@ static int b_plan_coarse(gfd_batch_poisson* b, int pre, int post, int coarse,
             cg.lev[0].r = -1;
             off -= n0;
         }
+        // Chebyshev iteration on the coarsest level when its update array
+        // fits (the batch's inner cycles are the mixed schedule's)
+        {
+            CLevel& C = cg.lev[cg.nlev - 1];
+            const int nc = C.n[0] * C.n[1] * C.n[2];
+            double lo, hi;
+            C.cheb = 0;
+            C.dbuf = -1;
+            if (coarse_cheb_enabled() && (size_t)(off + nc) * 4 <= COARSE_LDS_MAX &&
+                cheb_bounds(C, &lo, &hi)) {
+                C.cheb = 1;
+                C.dbuf = off;
+                C.lam_lo = lo;
+                C.lam_hi = hi;
+                off += nc;
+            }
+        }
         cg.lds_elems = off;
         cg.rhs_off = b->lev[lc].st[g].off;
         cg.out_off = b->lev[lc].st[g].off;
@@ -1138,7 +1155,7 @@ static int b_plan_coarse(gfd_batch_poisson* b, int pre, int post, int coarse,
                                  s));
     GFD_CHECK_HIP(hipStreamSynchronize(s));
     b->clds = maxb;
-    std::copy(key, key + 3, b->ckey);
+    std::copy(key, key + 4, b->ckey);
     *changed = true;
     return OK;
 }

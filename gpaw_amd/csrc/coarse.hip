@@ -10,6 +10,10 @@
 // residual (c0 e + sum_nb) - r, and the K = 2 transfer weights in the
 // same association order (z innermost, then y, then x).
 #include "coarse.h"
+
+#include <algorithm>
+#include <cmath>
+
 #include "stencil_dev.h"
 
 namespace gfd {
@@ -155,6 +159,59 @@ __device__ void crelax(T* sm, const CLevel& L, const T* rl, int sweeps,
     }
 }
 
+// `steps` steps of Chebyshev iteration for A e = r with the Jacobi
+// preconditioner D = c0 (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1): with the eigenvalues of D^-1 A in
+// [lo, hi], theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma =
+// theta / delta, z = D^-1 (r - A e):
+//   d_0 = z_0 / theta,  rho_0 = 1 / sigma
+//   rho_k = 1 / (2 sigma - rho_{k-1}),
+//   d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) z_k,  e += d_k.
+// After m steps the error is p_m(D^-1 A) e_0 with p_m the scaled
+// Chebyshev polynomial, |p_m| <= 2 ((sqrt k - 1) / (sqrt k + 1))^m on
+// [lo, hi] (k = hi / lo), where m Jacobi sweeps give (1 - w lo)^m.
+template <typename T>
+__device__ void ccheb(T* sm, const CLevel& L, const T* rl, int steps,
+                      unsigned& cur, int lbit)
+{
+    typedef typename ScalarOf<T>::type S;
+    const int plane = L.n[1] * L.n[2];
+    const CWork w = cwork(plane, L.n[0]);
+    T* D = sm + L.dbuf;
+    const double theta = 0.5 * (L.lam_hi + L.lam_lo);
+    const double delta = 0.5 * (L.lam_hi - L.lam_lo);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    for (int s = 0; s < steps; s++) {
+        double ca = 0.0, cb = 1.0 / theta;
+        if (s > 0) {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            ca = rn * rho;
+            cb = 2.0 * rn / delta;
+            rho = rn;
+        }
+        const S a = (S)ca, bq = (S)cb;
+        const bool inb = (cur >> lbit) & 1u;
+        const T* src = sm + (inb ? L.b : L.a);
+        T* dst = sm + (inb ? L.a : L.b);
+        for (int q = w.q0; q < plane; q += w.qstep) {
+            CCol c;
+            ccol(L, q, c);
+            for (int i0 = w.x0; i0 < w.x1; i0++) {
+                const int p = i0 * plane + q;
+                const T acc = cneigh(src, L, c, q, i0, plane);
+                const T z = (rl[p] - acc) / (S)L.c0 - src[p];
+                const T dd = s > 0 ? fmac(bq * z, a, D[p]) : bq * z;
+                D[p] = dd;
+                dst[p] = src[p] + dd;
+            }
+            if (w.qstep == plane) break;
+        }
+        __syncthreads();
+        cur ^= 1u << lbit;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_coarse(const CGrid* __restrict__ grids, const T* __restrict__ rhs,
@@ -263,7 +320,10 @@ k_coarse(const CGrid* __restrict__ grids, const T* __restrict__ rhs,
     {
         const CLevel& L = G.lev[nl - 1];
         const T* rl = L.r >= 0 ? sm + L.r : r0;
-        crelax(sm, L, rl, L.post, cur, nl - 1);
+        if (L.cheb)
+            ccheb(sm, L, rl, L.post, cur, nl - 1);
+        else
+            crelax(sm, L, rl, L.post, cur, nl - 1);
     }
     // up
     for (int l = nl - 2; l >= 0; l--) {
@@ -331,6 +391,59 @@ k_coarse(const CGrid* __restrict__ grids, const T* __restrict__ rhs,
         T* o = out + G.out_off;
         for (int p = threadIdx.x; p < n; p += blockDim.x) o[p] = E[p];
     }
+}
+
+static bool g_cheb = true;
+
+void coarse_set_cheb(bool on) { g_cheb = on; }
+bool coarse_cheb_enabled() { return g_cheb; }
+
+bool cheb_bounds(const CLevel& L, double* lo, double* hi)
+{
+    if (L.R != 1 || L.c0 == 0.0) return false;
+    // lambda(theta) = 1 + sum_c g_c(theta_c), g_c = (cp e^{i theta} +
+    // cm e^{-i theta}) / c0 = 2 cp cos(theta) / c0 for cp = cm
+    double gmin[3], gnext[3], gmax[3];
+    for (int c = 0; c < 3; c++) {
+        const double cp = L.has[c] ? L.cp[c][0] : 0.0;
+        const double cm = L.has[c] ? L.cm[c][0] : 0.0;
+        if (cp != cm) return false;
+        const int n = L.n[c];
+        gmin[c] = 1e300;
+        gnext[c] = 1e300;
+        gmax[c] = -1e300;
+        for (int j = 0; j < n; j++) {
+            const double th = L.periodic[c] ? 2.0 * M_PI * j / n
+                                            : M_PI * (j + 1) / (n + 1);
+            const double g = 2.0 * cp * std::cos(th) / L.c0;
+            gmax[c] = std::max(gmax[c], g);
+            if (g < gmin[c]) {
+                gnext[c] = gmin[c];
+                gmin[c] = g;
+            } else if (g < gnext[c]) {
+                gnext[c] = g;
+            }
+        }
+    }
+    double smin = 1.0, smax = 1.0;
+    for (int c = 0; c < 3; c++) {
+        smin += gmin[c];
+        smax += gmax[c];
+    }
+    double l0 = smin;
+    if (l0 <= 1e-9 * smax) {
+        // the constant mode of a periodic operator (eigenvalue 0): the
+        // smallest nonzero one moves one axis to its next mode
+        l0 = 1e300;
+        for (int c = 0; c < 3; c++)
+            if (gnext[c] < 1e299) l0 = std::min(l0, smin + gnext[c] - gmin[c]);
+    }
+    if (!(l0 > 0.0) || !(smax > l0) || l0 >= 1e299) return false;
+    // rounding and non-exact modes: a lower bound too low only slows the
+    // iteration, an upper bound too low lets the top modes grow
+    *lo = 0.9 * l0;
+    *hi = 1.05 * smax;
+    return true;
 }
 
 int launch_coarse(const CGrid* d_grids, int G, size_t lds_bytes, int dtype,

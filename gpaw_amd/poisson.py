@@ -443,8 +443,9 @@ class FDPoissonSolver(BasePoissonSolver):
         fp64_inner: the same cycles in FP64 (isolates what FP32 gains).
 
         levels: depth of the FP32 hierarchy (None: all levels of the FP64
-        one).  The coarsest FP32 level gets coarse_sweeps Jacobi sweeps, so
-        a shallow hierarchy leaves smooth error behind and needs more
+        one).  The coarsest FP32 level gets coarse_sweeps Jacobi sweeps
+        (Chebyshev steps when it is part of the LDS-resident coarse solve),
+        so a shallow hierarchy leaves smooth error behind and needs more
         refinement steps.  Not for the Mehrstellen pair (its right-hand
         side is B rho)."""
         if self.B is not None:

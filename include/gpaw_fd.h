@@ -184,7 +184,9 @@ int gfd_poisson_set_graph(gfd_poisson* p, int on);
  * residual r = L phi - rho, each step solving L e = r with `inner` FP32
  * V-cycles over `levels` levels (pre / post Jacobi sweeps per level,
  * coarse_sweeps at the coarsest, weights w0 at level 0 and wc below; -1
- * keeps the FDPoissonSolver value) and phi -= e; stops when
+ * keeps the FDPoissonSolver value; in the LDS-resident coarse solve the
+ * coarsest level's coarse_sweeps are Chebyshev steps, see "coarse_cheb")
+ * and phi -= e; stops when
  * |r| / |rho| <= tol_rel.  Returns GFD_ECONVERGE after max_outer steps.
  * inner_fp64 != 0 runs the same inner cycles (levels, sweeps, weights,
  * LDS coarse solve) on FP64 correction arrays instead, so that the two
@@ -256,7 +258,8 @@ int gfd_batch_poisson_transfer(gfd_batch_poisson* b, int level,
                                void* out, const void* base, void* stream);
 
 /* FP32 V-cycles (pre / post Jacobi sweeps, coarse_sweeps on the coarsest
- * level) inside FP64 refinement until |L phi_g - rho_g| / |rho_g| <=
+ * level: Chebyshev steps in the LDS-resident coarse solve, see
+ * "coarse_cheb") inside FP64 refinement until |L phi_g - rho_g| / |rho_g| <=
  * tol_rel for every grid; phis[g] in/out and rhos[g] are float64 device
  * arrays of grid g's fine shape.  use_graph replays one captured
  * refinement step.  relres[g] receives the final relative residuals;
@@ -439,7 +442,11 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * "coarse_lds" (0/1, default 1) runs the coarse levels of the Poisson
  * V-cycles that fit in one workgroup's LDS as a single launch;
  * "coarse_max_points" (default and 0: 4096) caps the points of its
- * first level;
+ * first level; "coarse_cheb" (0/1, default 1) makes the coarsest level of
+ * that solve in the mixed-precision schedules (single grid and batched)
+ * run Chebyshev iteration on D^-1 A (eigenvalue bounds from the radius-1
+ * stencil's Fourier / sine modes) instead of Jacobi sweeps; the reference
+ * schedule keeps Jacobi;
  * "xfer_passes" (0/1) runs K = 2 transfers as three 1-D passes instead of
  * the fused kernel; "xfer_march" (0/1, default 1) lets transfers use the
  * x-marching fused kernel (off: one point per thread for K = 2, three 1-D

@@ -202,7 +202,7 @@ def test_uneven_48_all_problems():
     say('48 uneven problems: tune')
     b.tune()
     phis = [torch.zeros_like(r) for r in rhos]
-    n, rel = b.solve_mixed(phis, rhos, 1e-10, coarse_sweeps=64)
+    n, rel = b.solve_mixed(phis, rhos, 1e-10, coarse_sweeps=16)
     say(f'  batched: {n} refinement steps, max relres {rel.max():.3g}')
     assert (rel <= 1e-10).all()
     worst = 0.0
