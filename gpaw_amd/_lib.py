@@ -69,6 +69,10 @@ SIGNATURES = {
     'gfd_batch_poisson_tiles': (_i, [_vp, _i, _ip, _ip, _ip,
                                      ctypes.POINTER(_l)]),
     'gfd_batch_poisson_tile_info': (_i, [_vp, _i, ctypes.POINTER(_l), _i]),
+    'gfd_coarse_cheb_bounds': (_i, [_ip, _ip, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
     'gfd_batch_poisson_apply': (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     'gfd_batch_poisson_transfer': (_i, [_vp, _i, _i, _i, _vp, _vp, _vp,
                                         _vp]),

@@ -9,6 +9,7 @@
 // is that of the global kernels: Jacobi (1 - w) e + w (r - sum_nb) / c0,
 // residual (c0 e + sum_nb) - r, and the K = 2 transfer weights in the
 // same association order (z innermost, then y, then x).
+#include "../../include/gpaw_fd.h"
 #include "coarse.h"
 
 #include <algorithm>
@@ -481,3 +482,31 @@ int launch_coarse(const CGrid* d_grids, int G, size_t lds_bytes, int dtype,
 }
 
 }  // namespace gfd
+
+extern "C" int gfd_coarse_cheb_bounds(const int* n, const int* periodic,
+                                      const double* cp, double c0,
+                                      double* lo, double* hi)
+{
+    if (!n || !periodic || !cp || !lo || !hi) {
+        gfd::set_error("gfd_coarse_cheb_bounds: null argument");
+        return gfd::EINVAL_;
+    }
+    gfd::CLevel L{};
+    L.R = 1;
+    L.c0 = c0;
+    for (int c = 0; c < 3; c++) {
+        if (n[c] < 1) {
+            gfd::set_error("gfd_coarse_cheb_bounds: bad shape");
+            return gfd::EINVAL_;
+        }
+        L.n[c] = n[c];
+        L.periodic[c] = periodic[c];
+        L.has[c] = cp[c] != 0.0;
+        L.cp[c][0] = L.cm[c][0] = cp[c];
+    }
+    if (!gfd::cheb_bounds(L, lo, hi)) {
+        gfd::set_error("gfd_coarse_cheb_bounds: no bounds for this stencil");
+        return gfd::ENOTSUP_;
+    }
+    return gfd::OK;
+}

@@ -465,6 +465,18 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 
+/* Eigenvalue bounds [lo, hi] that the LDS-resident coarse solve's
+ * Chebyshev iteration uses on a level with the radius-1 axis stencil
+ * c0 + cp[c] (e_{+c} + e_{-c}) on an n[0] x n[1] x n[2] grid (periodic[c]
+ * or zero boundary): the extreme nonzero eigenvalues of D^-1 A, 0.9 x the
+ * lowest and 1.05 x the highest.  No counterpart in the reference (the
+ * reference smooths the coarsest level with Jacobi); exported so that the
+ * bounds can be checked against dense eigenvalues.  GFD_ENOTSUP when no
+ * bounds apply. */
+int gfd_coarse_cheb_bounds(const int* n, const int* periodic,
+                           const double* cp, double c0, double* lo,
+                           double* hi);
+
 /* The full-row tiles the tuner times for a stencil radius, data type
  * (0 f64, 1 c128, 2 f32, 3 c64) and grid shape, besides its heuristic one:
  * up to max_codes of them are written to codes in the "xr_tile" encoding
