@@ -652,10 +652,12 @@ void batch_force_tile(int nt, int py, int zmax, int xc, int pf)
     g_bforce.pf = pf;
 }
 
-// Candidate tile shapes of both kernels, timed with a PLAIN pass over the
-// level (dtype 4: the FP64 data of the FP64 residual).
-static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
-                      const void* in, void* out, hipStream_t s)
+// Candidate tile shapes of both kernels, timed on the level's dominant
+// pass: the FP32 Jacobi sweep (epi EPI_RELAX, dtype 2) on the FP32 levels,
+// the FP64 residual with FP32 output (EPI_RESID, dtype 4) on the FP64 one.
+static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype, int epi,
+                      const void* in, const void* aux, void* out,
+                      hipStream_t s)
 {
     std::vector<BTile> cand;
     if (g_bforced) {
@@ -709,12 +711,10 @@ static int tune_level(gfd_batch_poisson* b, BLevel& lv, int dtype,
     for (const BTile& t : cand) {
         if (!fill_tiles(lv, t)) continue;
         if ((st = upload_level(lv, s)) != OK) break;
-        const int ep = EPI_PLAIN;
-        const int dt = dtype == 4 ? 0 : dtype;
-        if ((st = bst(lv, dt, ep, in, out, nullptr, nullptr, s)) != OK) break;
+        if ((st = bst(lv, dtype, epi, in, out, aux, nullptr, s)) != OK) break;
         (void)hipEventRecord(e0, s);
         for (int r = 0; r < 3 && st == OK; r++)
-            st = bst(lv, dt, ep, in, out, nullptr, nullptr, s);
+            st = bst(lv, dtype, epi, in, out, aux, nullptr, s);
         (void)hipEventRecord(e1, s);
         (void)hipEventSynchronize(e1);
         if (st != OK) break;
@@ -1011,10 +1011,14 @@ static int b_tune(gfd_batch_poisson* b, hipStream_t s)
     for (int l = 0; l < b->L; l++) {
         BLevel& lv = b->lev[l];
         GFD_TRY(vec_zero_bytes(b->e32[l].p, 4 * lv.total, s));
-        GFD_TRY(tune_level(b, lv, 2, b->e32[l].p, b->t32[l].p, s));
+        GFD_TRY(vec_zero_bytes(b->r32[l].p, 4 * lv.total, s));
+        GFD_TRY(tune_level(b, lv, 2, EPI_RELAX, b->e32[l].p, b->r32[l].p,
+                           b->t32[l].p, s));
     }
-    GFD_TRY(vec_zero(( double*)b->phi.p, b->lev[0].total, s));
-    GFD_TRY(tune_level(b, b->l0_64, 0, b->phi.p, b->rho.p, s));
+    GFD_TRY(vec_zero((double*)b->phi.p, b->lev[0].total, s));
+    GFD_TRY(vec_zero((double*)b->rho.p, b->lev[0].total, s));
+    GFD_TRY(tune_level(b, b->l0_64, 4, EPI_RESID, b->phi.p, b->rho.p,
+                       b->r32[0].p, s));
     long mt = 0;
     for (auto& lv : b->lev) mt = std::max(mt, lv.ntiles);
     mt = std::max(mt, b->l0_64.ntiles);
