@@ -1844,6 +1844,20 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         batch_force_tile(value ? nt : 0, py, zmax, xc, pf);
         return OK;
     }
+    if (std::strcmp(key, "batch_xfer") == 0) {
+        const long nt = value & 0xfff, py = (value >> 12) & 0xf;
+        const long zmax = (value >> 16) & 0xfff, xc = (value >> 28) & 0xfff;
+        if (value < 0 || (value > 2 && value < 64) ||
+            (value >= 64 && (nt < 64 || nt > 512 || nt % 64 ||
+                             (py != 1 && py != 2 &&
+                                                    py != 4) ||
+                             zmax < 1 || xc < 1))) {
+            set_error("gfd_set_tuning: batch_xfer 0, 1, 2 or a tile code");
+            return EINVAL_;
+        }
+        batch_set_xfer(value);
+        return OK;
+    }
     if (std::strcmp(key, "batch_kernel") == 0) {
         if (value < 0 || value > 2) {
             set_error("gfd_set_tuning: batch_kernel 0, 1 or 2");

@@ -60,6 +60,72 @@ __device__ inline bool bwrap(int& i, int n, int periodic)
     return true;
 }
 
+// One grid pair of a batched K = 2 transfer.
+struct BXf {
+    long in_off, out_off;
+    long p0;           // first output point of the grid in the launch
+    int ni[3], no[3];
+    int off[3], skip0[3];
+    int periodic[3];
+    double alpha;      // XEPI_SCALE2: out2 = alpha out (per grid)
+    // x-march tiles over the output grid (bxfer_march.hip): first tile,
+    // tiles along z and y, output planes per tile, z width, row segments
+    long tile0;
+    int tz, ty, xc, mz, mseg;
+};
+
+struct BXArgs {
+    const BXf* g;
+    int G;
+    long total;
+    const void* in;
+    void* out;
+    const void* base;  // XEPI_SUBFROM: out = base - T(in)
+    void* out2;        // XEPI_SCALE2
+    int negate;        // XEPI_SCALE2: out = -T(in)
+    long ntiles;       // x-march launch: tiles of all grids
+};
+
+// Buffer resources: raw loads and stores through a resource whose
+// base is the start of a plane and whose size is the plane's bytes (0 for
+// a zero plane); a byte offset at or past the size reads 0 and drops a
+// store, which is how zero-boundary neighbours, points outside the grid
+// and lanes without an output are masked without branches.
+constexpr unsigned BXM_OOB = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t brsrc(const void* base,
+                                               unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base),
+                                             (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+template <typename T>
+__device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+    } else {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+    }
+}
+
+template <typename T>
+__device__ inline void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __builtin_bit_cast(unsigned, v), r, (int)voff, 0, 0);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r,
+                                              (int)voff, 0, 0);
+    }
+}
+
 // x-march batched stencil (bxmarch.hip): 1-D blocks of `nt` threads, each
 // grid's tile TZ = mz columns by mseg * py rows, pf planes (1, 2) of
 // loads in flight; `lds` dynamic bytes.
@@ -68,9 +134,20 @@ constexpr int BXM_HMAX = 2;     // halo elements per thread per plane
 constexpr int BXM_MAXT = 512;
 // 0: timed choice of the two kernels, 1: register tiles, 2: x-march
 void batch_set_kernel(int kind);
+// batched transfers tuned afterwards: 0 timed choice, 1 one point per
+// thread, 2 x-march (timed tile), or a forced x-march tile nt | py << 12 |
+// zmax << 16 | xc << 28 (nt >= 64)
+void batch_set_xfer(long code);
 // x-march tile used by every level tuned afterwards (nt 0: timed choice)
 void batch_force_tile(int nt, int py, int zmax, int xc, int pf);
 int launch_bxmarch(const BSArgs& a, int R, int py, int pf, int nt,
                    size_t lds, int dtype, int epi, hipStream_t s);
+
+// x-march batched K = 2 transfers (bxfer_march.hip): one block of `nt`
+// threads per tile of each grid's output (table fields tile0 .. mseg), py
+// output rows per thread; dtype 0 FP64, 2 FP32, 4 FP32 from FP64 input
+// (restriction); epilogue xe of transfer.h.
+int launch_bxfer_march(const BXArgs& a, bool interp, int py, int nt,
+                       int dtype, int xe, hipStream_t s);
 
 }  // namespace gfd

@@ -239,27 +239,6 @@ k_bnorms(const double* __restrict__ partial, const BSt* __restrict__ g, int G,
     if (threadIdx.x == 0) out[gi] = red[0];
 }
 
-// One grid pair of a batched K = 2 transfer.
-struct BXf {
-    long in_off, out_off;
-    long p0;           // first output point of the grid in the launch
-    int ni[3], no[3];
-    int off[3], skip0[3];
-    int periodic[3];
-    double alpha;      // XEPI_SCALE2: out2 = alpha out (per grid)
-};
-
-struct BXArgs {
-    const BXf* g;
-    int G;
-    long total;
-    const void* in;
-    void* out;
-    const void* base;  // XEPI_SUBFROM: out = base - T(in)
-    void* out2;        // XEPI_SCALE2
-    int negate;        // XEPI_SCALE2: out = -T(in)
-};
-
 __device__ inline int bfind_point(const BXf* g, int G, long p)
 {
     int lo = 0, hi = G - 1;
@@ -392,10 +371,17 @@ struct BLevel {
     DevBuf d_st;
 };
 
+// x-march transfer tiles of one direction (nt 0: one point per thread)
+struct BXTile {
+    int nt = 0, py = 1, zmax = 64, xc = 8;
+    long ntiles = 0;
+};
+
 struct BPair {                      // transfer from level l to l + 1
     std::vector<BXf> rst, itp;
     long nr = 0, ni = 0;            // output points
     DevBuf d_rst, d_itp;
+    BXTile trst, titp;
 };
 
 }  // namespace gfd
@@ -575,6 +561,33 @@ static int bst(const BLevel& lv, int dtype, int epi, const void* in,
     return ENOTSUP_;
 }
 
+// x-march tiles of every grid's output for tile shape t; false if a grid's
+// input plane is too large for 32-bit offsets
+static bool fill_xtiles(std::vector<BXf>& v, BXTile& t)
+{
+    long t0 = 0;
+    bool ok = true;
+    for (BXf& x : v) {
+        if (t.nt <= 0) {
+            x.tile0 = 0;
+            x.tz = x.ty = x.xc = x.mz = x.mseg = 1;
+            continue;
+        }
+        const int zw = std::min(t.zmax, t.nt);
+        x.tz = (x.no[2] + zw - 1) / zw;
+        x.mz = (x.no[2] + x.tz - 1) / x.tz;
+        x.mseg = std::max(1, std::min(t.nt / x.mz, (x.no[1] + t.py - 1) / t.py));
+        x.ty = (x.no[1] + x.mseg * t.py - 1) / (x.mseg * t.py);
+        x.xc = std::min(t.xc, x.no[0]);
+        x.tile0 = t0;
+        t0 += (long)((x.no[0] + x.xc - 1) / x.xc) * x.ty * x.tz;
+        const long pl = std::max((long)x.ni[1] * x.ni[2], (long)x.no[1] * x.no[2]);
+        if (pl * 8 >= 0x7fffffffL) ok = false;
+    }
+    t.ntiles = t0;
+    return ok && t0 <= 0x7fffffffL;
+}
+
 template <typename T, typename TI, bool I, int XE>
 static void bxfer_launch(const BXArgs& a, long nb, hipStream_t s)
 {
@@ -600,6 +613,11 @@ static int bxfer(const BPair& pr, bool interp, int dtype, const void* in,
     a.negate = negate;
     if (xe < 0) xe = interp ? XEPI_SUBFROM : XEPI_PLAIN;
     if (a.total <= 0) return OK;
+    const BXTile& xt = interp ? pr.titp : pr.trst;
+    if (xt.nt > 0) {
+        a.ntiles = xt.ntiles;
+        return launch_bxfer_march(a, interp, xt.py, xt.nt, dtype, xe, s);
+    }
     const long nb = std::min((a.total + 255) / 256, (long)num_cu * 16);
     if (dtype == 0 && !interp && xe == XEPI_PLAIN)
         bxfer_launch<double, double, false, XEPI_PLAIN>(a, nb, s);
@@ -1005,6 +1023,99 @@ static int b_alloc(gfd_batch_poisson* b, hipStream_t s)
     return OK;
 }
 
+static int g_bxkind = 0;   // 0: timed choice, 1: one point per thread,
+                          // 2: x-march (timed), 3: the forced x-march tile
+static BXTile g_bxforce;
+
+void batch_set_xfer(long code)
+{
+    g_bxkind = code >= 64 ? 3 : (int)code;
+    if (code >= 64) {
+        g_bxforce = BXTile();
+        g_bxforce.nt = (int)(code & 0xfff);
+        g_bxforce.py = (int)((code >> 12) & 0xf);
+        g_bxforce.zmax = (int)((code >> 16) & 0xfff);
+        g_bxforce.xc = (int)((code >> 28) & 0xfff);
+    }
+}
+
+static int upload_xtable(const std::vector<BXf>& v, DevBuf& d, hipStream_t s)
+{
+    GFD_TRY(d.ensure(sizeof(BXf) * v.size(), s));
+    GFD_CHECK_HIP(hipMemcpyAsync(d.p, v.data(), sizeof(BXf) * v.size(),
+                                 hipMemcpyHostToDevice, s));
+    return hipStreamSynchronize(s) == hipSuccess ? OK : EHIP;
+}
+
+// Transfer kernel and tile per direction of pair l, timed on the V-cycle's
+// FP32 passes (restriction PLAIN, interpolation SUBFROM).
+static int tune_pair(gfd_batch_poisson* b, int l, hipStream_t s)
+{
+    BPair& pr = b->pair[l];
+    std::vector<BXTile> cand;
+    if (g_bxkind == 0 || g_bxkind == 1) cand.push_back(BXTile());
+    if (g_bxkind == 3) cand.push_back(g_bxforce);
+    if (g_bxkind == 0 || g_bxkind == 2)
+        for (int nt : {256, 512})
+            for (int py : {1, 2, 4})
+                for (int zmax : {64, 128})
+                    for (int xc : {4, 8, 16}) {
+                        BXTile t;
+                        t.nt = nt;
+                        t.py = py;
+                        t.zmax = zmax;
+                        t.xc = xc;
+                        cand.push_back(t);
+                    }
+    hipEvent_t e0, e1;
+    GFD_CHECK_HIP(hipEventCreate(&e0));
+    GFD_CHECK_HIP(hipEventCreate(&e1));
+    int st = OK;
+    for (int interp = 0; interp < 2 && st == OK; interp++) {
+        std::vector<BXf>& v = interp ? pr.itp : pr.rst;
+        DevBuf& dv = interp ? pr.d_itp : pr.d_rst;
+        BXTile& cur = interp ? pr.titp : pr.trst;
+        const void* in = interp ? b->e32[l + 1].p : b->s32[l].p;
+        void* out = interp ? b->t32[l].p : b->r32[l + 1].p;
+        const void* base = interp ? b->e32[l].p : nullptr;
+        float best = 1e30f;
+        BXTile bt;
+        bool found = false;
+        for (BXTile t : cand) {
+            if (!fill_xtiles(v, t)) continue;
+            if ((st = upload_xtable(v, dv, s)) != OK) break;
+            cur = t;
+            if ((st = bxfer(pr, interp, 2, in, out, base, b->num_cu, s)) != OK)
+                break;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && st == OK; r++)
+                st = bxfer(pr, interp, 2, in, out, base, b->num_cu, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            if (st != OK) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                bt = t;
+                found = true;
+            }
+        }
+        if (st != OK) break;
+        if (!found) {
+            set_error("batched transfer: no tile fits level %d", l);
+            st = ENOTSUP_;
+            break;
+        }
+        fill_xtiles(v, bt);
+        cur = bt;
+        st = upload_xtable(v, dv, s);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
 static int b_tune(gfd_batch_poisson* b, hipStream_t s)
 {
     GFD_TRY(b_alloc(b, s));
@@ -1019,6 +1130,7 @@ static int b_tune(gfd_batch_poisson* b, hipStream_t s)
     GFD_TRY(vec_zero((double*)b->rho.p, b->lev[0].total, s));
     GFD_TRY(tune_level(b, b->l0_64, 4, EPI_RESID, b->phi.p, b->rho.p,
                        b->r32[0].p, s));
+    for (int l = 0; l + 1 < b->L; l++) GFD_TRY(tune_pair(b, l, s));
     long mt = 0;
     for (auto& lv : b->lev) mt = std::max(mt, lv.ntiles);
     mt = std::max(mt, b->l0_64.ntiles);

@@ -32,46 +32,6 @@
 
 namespace gfd {
 
-// Buffer resources: raw loads and stores through a resource whose
-// base is the start of a plane and whose size is the plane's bytes (0 for
-// a zero plane); a byte offset at or past the size reads 0 and drops a
-// store, which is how zero-boundary neighbours, points outside the grid
-// and lanes without an output are masked without branches.
-constexpr unsigned BXM_OOB = 0x80000000u;
-
-__device__ inline __amdgpu_buffer_rsrc_t brsrc(const void* base,
-                                               unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base),
-                                             (short)0, (int)bytes,
-                                             0x00020000);
-}
-
-template <typename T>
-__device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff)
-{
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(
-            T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
-    } else {
-        return __builtin_bit_cast(
-            T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
-    }
-}
-
-template <typename T>
-__device__ inline void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned voff)
-{
-    if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(
-            __builtin_bit_cast(unsigned, v), r, (int)voff, 0, 0);
-    } else {
-        typedef unsigned u2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r,
-                                              (int)voff, 0, 0);
-    }
-}
-
 template <typename T, typename TO, int R, int PY, int PF, int EPI>
 __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
 {

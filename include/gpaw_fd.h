@@ -461,7 +461,11 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * 12 | zmax << 16 | xc << 28 | pf << 40, 0 timed choice) makes every level
  * tuned afterwards use that x-march tile: nt threads per block (64 .. 512,
  * multiple of 64), py rows per thread (1, 2, 4), z parts of at most zmax
- * columns, xc x planes per tile, pf planes of loads in flight (1, 2).
+ * columns, xc x planes per tile, pf planes of loads in flight (1, 2);
+ * "batch_xfer" (0 timed choice, 1 one output point per thread, 2 x-march
+ * tiles, or a tile code nt | py << 12 | zmax << 16 | xc << 28 with nt >= 64
+ * forcing that x-march tile) sets the batched transfers of batches tuned
+ * afterwards.
  * Returns GFD_EINVAL for unknown keys or values. */
 int gfd_set_tuning(const char* key, long value);
 

@@ -69,3 +69,15 @@ def batch_kernel(name='march'):
         yield
     finally:
         lib.gfd_set_tuning(b'batch_kernel', 0)
+
+
+@contextlib.contextmanager
+def batch_xfer(code):
+    """Batched transfers of batches tuned while active: 1 one point per
+    thread, 2 x-march tiles, or a forced x-march tile code."""
+    lib = _lib.load()
+    _lib.check(lib.gfd_set_tuning(b'batch_xfer', code), 'batch_xfer')
+    try:
+        yield
+    finally:
+        lib.gfd_set_tuning(b'batch_xfer', 0)

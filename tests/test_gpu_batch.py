@@ -8,7 +8,7 @@ import oracle
 from gpaw_amd import workloads
 from gpaw_amd.batch_poisson import BatchedPoissonSolver
 from gpaw_amd.grid_descriptor import GridDescriptor
-from gpu_util import batch_kernel, dev, host, say
+from gpu_util import batch_kernel, batch_xfer, dev, host, say
 from helpers import relerr, spec, tspec
 
 pytestmark = pytest.mark.gpu
@@ -86,9 +86,22 @@ def _check_apply(b):
             assert relerr(host(got).astype(float), ref) < 2e-6
 
 
-def test_transfers():
+def _xcode(nt, py, zmax, xc):
+    return nt | py << 12 | zmax << 16 | xc << 28
+
+
+XFER_MODES = [1, 2, _xcode(256, 1, 64, 4), _xcode(512, 2, 16, 3),
+              _xcode(128, 4, 128, 16), _xcode(64, 1, 8, 5)]
+
+
+@pytest.mark.parametrize('mode', XFER_MODES)
+def test_transfers(mode):
+    """One point per thread (1), the timed x-march tile (2) and forced
+    x-march tiles (z parts narrower than a row, ragged x chunks)."""
     import torch
-    b = batch()
+    with batch_xfer(mode):
+        b = batch()
+        b.tune()
     rng = np.random.default_rng(22)
     for level in range(b.L - 1):
         rs = [s.restrictors[level] for s in b.solvers]
@@ -157,7 +170,8 @@ def _precond_ref(kin, rst, itp, res, step):
                              tspec(itp[1]), res, step)
 
 
-def test_batched_preconditioner(kernel):
+@pytest.mark.parametrize('xfer', [0, 2])
+def test_batched_preconditioner(kernel, xfer):
     """The FP32 chain on five uneven grids in one set of launches, grid by
     grid against the FP64 oracle (float32 rounding over ~10 passes: bar
     2e-6, as for the single-grid FP32 chain) and against the single-grid
@@ -168,7 +182,10 @@ def test_batched_preconditioner(kernel):
     rng = np.random.default_rng(23)
     gds = [GridDescriptor(N, np.array(N) * (0.3 + 0.02 * i), pbc)
            for i, (N, pbc) in enumerate(SHAPES)]
-    P = BatchedPreconditioner(gds)
+    with batch_xfer(xfer):
+        P = BatchedPreconditioner(gds)
+        # the first call tunes the batch (transfers included)
+        P([dev(np.zeros(tuple(gd.n_c))) for gd in gds])
     assert len(set(P.steps)) > 1  # per-grid steps
     res = [rng.uniform(-1, 1, tuple(gd.n_c)) for gd in gds]
     say('batched preconditioner', len(gds), 'grids')
