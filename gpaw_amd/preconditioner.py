@@ -176,6 +176,32 @@ class BatchedPreconditioner:
         except Exception:
             pass
 
+    @property
+    def npoints(self):
+        """Points of all grids: the length of the packed layout."""
+        return int(sum(int(np.prod(gd.n_c)) for gd in self.gds))
+
+    def pack(self, arrays):
+        """The per-grid arrays back to back (grid g after grids 0..g-1),
+        the layout apply_packed works on."""
+        return torch.cat([a.reshape(-1) for a in arrays])
+
+    def apply_packed(self, res, out):
+        """The chain on residuals already in the packed layout (float64,
+        npoints long) into ``out`` (same layout, not aliasing res): no
+        gathering or scattering of per-grid arrays."""
+        n = self.npoints
+        for name, t in (('res', res), ('out', out)):
+            if (t.dtype != torch.float64 or t.numel() != n or
+                    not t.is_contiguous()):
+                raise TypeError('%s: contiguous float64 array of %d points'
+                                % (name, n))
+        steps = (_lib.ctypes.c_double * len(self.steps))(*self.steps)
+        _lib.check(_lib.load().gfd_batch_precond32(
+            self._handle(), steps, ptr(res), ptr(out), stream_handle()),
+            'BatchedPreconditioner')
+        return out
+
     def __call__(self, residuals, out=None):
         """residuals: one float64 device array per grid (its shape);
         returns (or fills ``out`` with) the preconditioned arrays."""
@@ -185,12 +211,9 @@ class BatchedPreconditioner:
             if tuple(r.shape) != tuple(gd.n_c) or r.dtype != torch.float64:
                 raise TypeError('residuals: float64 arrays of the grid '
                                 'shapes')
-        flat = torch.cat([r.reshape(-1) for r in residuals])
+        flat = self.pack(residuals)
         res_out = torch.empty_like(flat)
-        steps = (_lib.ctypes.c_double * len(self.steps))(*self.steps)
-        _lib.check(_lib.load().gfd_batch_precond32(
-            self._handle(), steps, ptr(flat), ptr(res_out),
-            stream_handle()), 'BatchedPreconditioner')
+        self.apply_packed(flat, res_out)
         if out is None:
             out = [None] * len(self.gds)
         i = 0

@@ -204,3 +204,11 @@ def test_batched_preconditioner(kernel, xfer):
     out2 = P([dev(r) for r in res2])
     for g, o in enumerate(out2):
         assert relerr(host(o), 2 * host(out[g])) < 1e-6
+    # the packed layout gives the same bits as the per-grid call
+    import torch
+    flat = P.pack([dev(r) for r in res])
+    fout = torch.empty_like(flat)
+    P.apply_packed(flat, fout)
+    assert (host(fout) == host(P.pack(out))).all()
+    with pytest.raises(TypeError):
+        P.apply_packed(flat[1:], fout)
