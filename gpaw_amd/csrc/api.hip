@@ -1170,14 +1170,35 @@ static int plan_coarse(gfd_poisson* p, const Schedule& sc, int dtype,
 }
 
 // gpaw/poisson.py:500-526 for one level; *cur / *alt hold phi_l.
+// zero: phi_l starts at 0 (its contents are not read).  On the mixed
+// schedules' levels the first Jacobi sweep from 0 is then the scaled
+// right-hand side (one pass without the stencil); the reference schedule
+// zeroes phi_l and sweeps as the reference does.
 static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
-                  void** cur, void** alt, const void* rho, hipStream_t s)
+                  void** cur, void** alt, const void* rho, hipStream_t s,
+                  bool zero = false)
 {
     gfd_operator* op = p->ops[l];
     const int dt = lv.dtype;
     const size_t es = dt == 2 ? 4 : 8;
+    const bool mixed = &lv != &p->lv;
+    // sweeps of the first smoothing stage from phi_l = 0
+    auto relax_from = [&](int n, int& done) -> int {
+        done = 0;
+        if (!zero) return OK;
+        zero = false;
+        const bool jacobi = !(p->method == 1 && dt == 0);
+        if (mixed && jacobi && n > 0) {
+            done = 1;
+            return vec_relax0(*cur, rho, op->ng(), sched_w(sc, l),
+                              op->coef[0], dt, s);
+        }
+        return vec_zero_bytes(*cur, es * op->ng(), s);
+    };
     if (l < sc.coarsest) {
-        GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_pre(sc, l),
+        int done = 0;
+        GFD_TRY(relax_from(sched_pre(sc, l), done));
+        GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_pre(sc, l) - done,
                            sched_w(sc, l), s, p->method));
         EpiArgs e{};
         e.aux = rho;
@@ -1196,8 +1217,8 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
             GFD_TRY(launch_coarse((const CGrid*)cs.dev.p, 1, cs.lds, dt,
                                   lv.rho[l + 1], c1, s));
         } else {
-            GFD_TRY(vec_zero_bytes(c1, es * p->ops[l + 1]->ng(), s));
-            GFD_TRY(vcycle(p, lv, sc, l + 1, &c1, &a1, lv.rho[l + 1], s));
+            GFD_TRY(vcycle(p, lv, sc, l + 1, &c1, &a1, lv.rho[l + 1], s,
+                           true));
         }
         XEpiArgs ce;
         ce.epi = XEPI_SUBFROM;
@@ -1206,7 +1227,9 @@ static int vcycle(gfd_poisson* p, Levels& lv, const Schedule& sc, int l,
                         p->xscratch.p));
         std::swap(*cur, *alt);
     }
-    GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_post(sc, l),
+    int done = 0;
+    GFD_TRY(relax_from(sched_post(sc, l), done));
+    GFD_TRY(relax_pp_t(op, dt, cur, alt, rho, sched_post(sc, l) - done,
                        sched_w(sc, l), s, p->method));
     return OK;
 }

@@ -219,6 +219,37 @@ __global__ void __launch_bounds__(256) k_bstencil(BSArgs a)
     }
 }
 
+// e = (w_g / c0_g) f on every grid g: a Jacobi sweep from e = 0 (the
+// x-march kernel's Jacobi update multiplies by the same FP32 w / c0)
+__global__ void __launch_bounds__(256)
+k_bscale0(const BSt* __restrict__ g, int G, long total,
+          const float* __restrict__ f, float* __restrict__ e)
+{
+    for (long b0 = (long)blockIdx.x * 1024; b0 < total;
+         b0 += (long)gridDim.x * 1024) {
+        int lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (g[mid].off <= b0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        int gi = lo;
+        float wc = (float)(g[gi].w / g[gi].c0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long i = b0 + k * 256 + threadIdx.x;
+            if (i >= total) break;
+            while (gi + 1 < G && g[gi + 1].off <= i) {
+                gi++;
+                wc = (float)(g[gi].w / g[gi].c0);
+            }
+            e[i] = wc * f[i];
+        }
+    }
+}
+
 // per-grid sums of the tile partials: one block per grid
 __global__ void __launch_bounds__(256)
 k_bnorms(const double* __restrict__ partial, const BSt* __restrict__ g, int G,
@@ -1150,30 +1181,45 @@ static int b_relax(gfd_batch_poisson* b, int l, void** cur, void** alt,
     return OK;
 }
 
-// FP32 V-cycle on e_l (zero on entry) for right-hand side r_l
+// n Jacobi sweeps from e = 0 (*cur is not read): the first is the scaled
+// right-hand side; n = 0 leaves e = 0
+static int b_relax0(gfd_batch_poisson* b, int l, void** cur, void** alt,
+                    const void* rhs, int n, hipStream_t s)
+{
+    const BLevel& lv = b->lev[l];
+    if (n <= 0) return vec_zero_bytes(*cur, 4 * lv.total, s);
+    const long nb = std::min((lv.total + 1023) / 1024, (long)b->num_cu * 16);
+    hipLaunchKernelGGL(k_bscale0, dim3((unsigned)std::max(1L, nb)), dim3(256),
+                       0, s, (const BSt*)lv.d_st.p, (int)lv.st.size(),
+                       lv.total, (const float*)rhs, (float*)*cur);
+    GFD_LAUNCH_CHECK();
+    return b_relax(b, l, cur, alt, rhs, n - 1, s);
+}
+
+// FP32 V-cycle on e_l for right-hand side r_l, from e_l = 0 (*cur is not
+// read)
 static int b_vcycle(gfd_batch_poisson* b, int l, void** cur, void** alt,
                     const void* rhs, int pre, int post, int coarse,
                     hipStream_t s)
 {
     if (l + 1 < b->L) {
-        GFD_TRY(b_relax(b, l, cur, alt, rhs, pre, s));
+        GFD_TRY(b_relax0(b, l, cur, alt, rhs, pre, s));
         GFD_TRY(bst(b->lev[l], 2, EPI_RESID, *cur, b->s32[l].p, rhs, nullptr,
                     s));
         GFD_TRY(bxfer(b->pair[l], false, 2, b->s32[l].p, b->r32[l + 1].p,
                       nullptr, b->num_cu, s));
         void* c1 = b->e32[l + 1].p;
         void* a1 = b->t32[l + 1].p;
-        GFD_TRY(vec_zero_bytes(c1, 4 * b->lev[l + 1].total, s));
         GFD_TRY(b_vcycle(b, l + 1, &c1, &a1, b->r32[l + 1].p, pre, post,
                          coarse, s));
         GFD_TRY(bxfer(b->pair[l], true, 2, c1, *alt, *cur, b->num_cu, s));
         std::swap(*cur, *alt);
         return b_relax(b, l, cur, alt, rhs, post, s);
     }
-    if (b->coarse)
+    if (b->coarse)  // starts from e = 0 in LDS
         return launch_coarse((const CGrid*)b->d_cgrids.p, b->G, b->clds, 2,
                              rhs, *cur, s);
-    return b_relax(b, l, cur, alt, rhs, coarse, s);
+    return b_relax0(b, l, cur, alt, rhs, coarse, s);
 }
 
 // CGrid of every grid for levels L-1 .. nlev_g - 1 (pre / post sweeps,
@@ -1284,7 +1330,6 @@ static int b_step(gfd_batch_poisson* b, int pre, int post, int coarse,
     const long n0 = b->lev[0].total;
     void* cur = b->e32[0].p;
     void* alt = b->t32[0].p;
-    GFD_TRY(vec_zero_bytes(cur, 4 * n0, s));
     GFD_TRY(b_vcycle(b, 0, &cur, &alt, b->r32[0].p, pre, post, coarse, s));
     GFD_TRY(vec_sub_f((double*)b->phi.p, (const float*)cur, n0, s));
     GFD_TRY(bst(b->l0_64, 4, EPI_RESID, b->phi.p, b->r32[0].p, b->rho.p,

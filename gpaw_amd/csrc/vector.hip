@@ -94,6 +94,17 @@ k_sub_d(double* __restrict__ a, const double* __restrict__ e, long n)
         a[i] -= e[i];
 }
 
+// out = w f / c0: a weighted Jacobi sweep from e = 0, in the arithmetic of
+// the stencil epilogue ((1 - w) 0 + w (f - 0) / c0)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_relax0(T* __restrict__ out, const T* __restrict__ f, long n, T w, T c0)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long)gridDim.x * 256)
+        out[i] = w * f[i] / c0;
+}
+
 static unsigned grid_for(long n)
 {
     long b = (n + 255) / 256;
@@ -132,6 +143,21 @@ int vec_sub_f(double* a, const float* e, long n, hipStream_t s)
 {
     if (n <= 0) return OK;
     hipLaunchKernelGGL(k_sub_f, dim3(grid_for(n)), dim3(256), 0, s, a, e, n);
+    GFD_LAUNCH_CHECK();
+    return OK;
+}
+
+int vec_relax0(void* out, const void* f, long n, double w, double c0,
+               int dtype, hipStream_t s)
+{
+    if (n <= 0) return OK;
+    if (dtype == 2)
+        hipLaunchKernelGGL(k_relax0<float>, dim3(grid_for(n)), dim3(256), 0,
+                           s, (float*)out, (const float*)f, n, (float)w,
+                           (float)c0);
+    else
+        hipLaunchKernelGGL(k_relax0<double>, dim3(grid_for(n)), dim3(256), 0,
+                           s, (double*)out, (const double*)f, n, w, c0);
     GFD_LAUNCH_CHECK();
     return OK;
 }
