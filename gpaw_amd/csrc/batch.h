@@ -126,6 +126,11 @@ __device__ inline void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned voff)
     }
 }
 
+// batched-only epilogue of the x-march kernel: a Jacobi sweep (FP32 data,
+// FP32 rhs in aux) whose result e is subtracted from the FP64 array `out`
+// at the output points (out -= e), in place of storing e
+constexpr int BEPI_RELAXSUB = 16;
+
 // x-march batched stencil (bxmarch.hip): 1-D blocks of `nt` threads, each
 // grid's tile TZ = mz columns by mseg * py rows, pf planes (1, 2) of
 // loads in flight; `lds` dynamic bytes.

@@ -1197,10 +1197,14 @@ static int b_relax0(gfd_batch_poisson* b, int l, void** cur, void** alt,
 }
 
 // FP32 V-cycle on e_l for right-hand side r_l, from e_l = 0 (*cur is not
-// read)
+// read).  phi (level 0 only, may be null): when the level's stencil runs
+// on x-march tiles, the last post-smoothing sweep subtracts its result
+// from the FP64 phi in the same pass and *sub is set (e_0 is then not
+// stored).
 static int b_vcycle(gfd_batch_poisson* b, int l, void** cur, void** alt,
                     const void* rhs, int pre, int post, int coarse,
-                    hipStream_t s)
+                    hipStream_t s, double* phi = nullptr,
+                    bool* sub = nullptr)
 {
     if (l + 1 < b->L) {
         GFD_TRY(b_relax0(b, l, cur, alt, rhs, pre, s));
@@ -1214,6 +1218,13 @@ static int b_vcycle(gfd_batch_poisson* b, int l, void** cur, void** alt,
                          coarse, s));
         GFD_TRY(bxfer(b->pair[l], true, 2, c1, *alt, *cur, b->num_cu, s));
         std::swap(*cur, *alt);
+        if (phi && post >= 1 && b->lev[l].tile.kind == 1) {
+            GFD_TRY(b_relax(b, l, cur, alt, rhs, post - 1, s));
+            GFD_TRY(bst(b->lev[l], 2, BEPI_RELAXSUB, *cur, phi, rhs, nullptr,
+                        s));
+            *sub = true;
+            return OK;
+        }
         return b_relax(b, l, cur, alt, rhs, post, s);
     }
     if (b->coarse)  // starts from e = 0 in LDS
@@ -1330,8 +1341,10 @@ static int b_step(gfd_batch_poisson* b, int pre, int post, int coarse,
     const long n0 = b->lev[0].total;
     void* cur = b->e32[0].p;
     void* alt = b->t32[0].p;
-    GFD_TRY(b_vcycle(b, 0, &cur, &alt, b->r32[0].p, pre, post, coarse, s));
-    GFD_TRY(vec_sub_f((double*)b->phi.p, (const float*)cur, n0, s));
+    bool sub = false;
+    GFD_TRY(b_vcycle(b, 0, &cur, &alt, b->r32[0].p, pre, post, coarse, s,
+                     (double*)b->phi.p, &sub));
+    if (!sub) GFD_TRY(vec_sub_f((double*)b->phi.p, (const float*)cur, n0, s));
     GFD_TRY(bst(b->l0_64, 4, EPI_RESID, b->phi.p, b->r32[0].p, b->rho.p,
                 (double*)b->partial.p, s));
     return bnorms(b->l0_64, (const double*)b->partial.p,

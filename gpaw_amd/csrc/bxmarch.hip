@@ -15,8 +15,10 @@
 // tiles are remapped so that neighbouring tiles of one grid share an XCD.
 //
 // Epilogues (per-grid c0, Jacobi weight and step from the table):
-// EPI_PLAIN, EPI_RELAX, EPI_RESID (+ per-tile sum of squares), EPI_UPDATE
-// and EPI_UPDW (FP32 data, FP64 aux and out), with the accumulation order
+// EPI_PLAIN, EPI_RELAX, EPI_RESID (+ per-tile sum of squares), EPI_UPDATE,
+// EPI_UPDW (FP32 data, FP64 aux and out) and BEPI_RELAXSUB (a Jacobi sweep
+// subtracted from an FP64 array: the last sweep of a refinement's
+// correction applied to phi in the same pass), with the accumulation order
 // of the register-tile kernel (z, then y, then x terms), so both kernels
 // give the same bits except for the Jacobi update, which multiplies by
 // w / c0 (rounded once per grid) where the other divides by c0.
@@ -38,6 +40,8 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
     // aux: right-hand side (RELAX, RESID, UPDATE) in T, FP64 for UPDW
     typedef typename std::conditional<EPI == EPI_UPDW, double, T>::type TA;
     constexpr bool AUX = EPI != EPI_PLAIN;
+    // RELAXSUB: the FP64 output array is read at the output points too
+    constexpr bool SUB = EPI == BEPI_RELAXSUB;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
 
@@ -144,6 +148,17 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
         for (int m = 0; m < BXM_HMAX; m++) dst[m] = bload<T>(r, h_off[m]);
     };
     // right-hand side of the output rows of plane x (inside the chunk)
+    // the FP64 array RELAXSUB updates, at the output points of plane x
+    auto ld_sub = [&](int x, double* dst) {
+        const __amdgpu_buffer_rsrc_t r = brsrc(
+            (const double*)a.out + d.off + (long)x * plane,
+            (unsigned)(plane * (long)sizeof(double)));
+#pragma unroll
+        for (int j = 0; j < PY; j++)
+            dst[j] = bload<double>(r, off_out[j] == BXM_OOB
+                                          ? BXM_OOB
+                                          : off_out[j] * 8u);
+    };
     auto ld_aux = [&](int x, TA* dst) {
         const __amdgpu_buffer_rsrc_t r = brsrc(
             gaux + (long)x * plane, (unsigned)(plane * (long)sizeof(TA)));
@@ -178,6 +193,7 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
     T qp[PF][PY];
     T hp[PF][BXM_HMAX];
     TA ap[AUX ? PF : 1][PY];
+    double sp[SUB ? PF : 1][PY];
 #pragma unroll
     for (int k = 0; k < 2 * R + 1; k++) ld_own(xs - R + k, q[k]);
 #pragma unroll
@@ -186,6 +202,7 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
         if (has_lds) ld_halo(x, hp[u]);
         ld_own(min(x + R + 1, xe + R - 1), qp[u]);
         if constexpr (AUX) ld_aux(x, ap[u]);
+        if constexpr (SUB) ld_sub(x, sp[u]);
     }
 
     double nrm = 0.0;
@@ -203,7 +220,7 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
             }
     }
 
-    auto body = [&](const int x, T* qpu, T* hpu, TA* apu) {
+    auto body = [&](const int x, T* qpu, T* hpu, TA* apu, double* spu) {
         T* buf = lds + (x & 1) * LSZ;
         if (has_lds) {
 #pragma unroll
@@ -231,6 +248,12 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
 #pragma unroll
             for (int j = 0; j < PY; j++) f[j] = apu[j];
             ld_aux(min(x + PF, xe - 1), apu);
+        }
+        double ph[PY];
+        if constexpr (SUB) {
+#pragma unroll
+            for (int j = 0; j < PY; j++) ph[j] = spu[j];
+            ld_sub(min(x + PF, xe - 1), spu);
         }
         const __amdgpu_buffer_rsrc_t ro = brsrc(
             gout + (long)x * plane, (unsigned)(plane * (long)sizeof(TO)));
@@ -267,7 +290,7 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
             T r;
             if constexpr (EPI == EPI_PLAIN) {
                 r = fmac(acc, c0, v);
-            } else if constexpr (EPI == EPI_RELAX) {
+            } else if constexpr (EPI == EPI_RELAX || SUB) {
                 r = fmac(w1 * v, wc, f[j] - acc);
             } else if constexpr (EPI == EPI_UPDATE || EPI == EPI_UPDW) {
                 const T qv = fmac(acc, c0, v) - (T)f[j];
@@ -277,10 +300,13 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
                 r = fmac(acc, c0, v) - f[j];
                 if (off_out[j] != BXM_OOB) nrm += (double)r * (double)r;
             }
-            bstore<TO>((TO)r, ro,
-                       off_out[j] == BXM_OOB
-                           ? BXM_OOB
-                           : off_out[j] * (unsigned)sizeof(TO));
+            const unsigned ob = off_out[j] == BXM_OOB
+                                    ? BXM_OOB
+                                    : off_out[j] * (unsigned)sizeof(TO);
+            if constexpr (SUB)
+                bstore<double>(ph[j] - (double)r, ro, ob);
+            else
+                bstore<TO>((TO)r, ro, ob);
         }
 #pragma unroll
         for (int k = 0; k < 2 * R; k++)
@@ -296,11 +322,12 @@ __global__ void __launch_bounds__(BXM_MAXT) k_bxmarch(BSArgs a)
     for (int x0 = xs; x0 < xm; x0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; u++)
-            body(x0 + u, qp[u], hp[u], ap[AUX ? u : 0]);
+            body(x0 + u, qp[u], hp[u], ap[AUX ? u : 0], sp[SUB ? u : 0]);
     }
 #pragma unroll
     for (int u = 0; u < PF; u++)
-        if (xm + u < xe) body(xm + u, qp[u], hp[u], ap[AUX ? u : 0]);
+        if (xm + u < xe)
+            body(xm + u, qp[u], hp[u], ap[AUX ? u : 0], sp[SUB ? u : 0]);
     if (EPI == EPI_RESID && a.partial) block_sum_store(nrm, a.partial, L);
 }
 
@@ -369,6 +396,9 @@ int launch_bxmarch(const BSArgs& a, int R, int py, int pf, int nt,
         return bxm_launch_r<float, float, EPI_PLAIN>(a, R, py, pf, nt, lds, s);
     if (dtype == 2 && epi == EPI_RELAX)
         return bxm_launch_r<float, float, EPI_RELAX>(a, R, py, pf, nt, lds, s);
+    if (dtype == 2 && epi == BEPI_RELAXSUB)
+        return bxm_launch_r<float, double, BEPI_RELAXSUB>(a, R, py, pf, nt,
+                                                          lds, s);
     if (dtype == 2 && epi == EPI_RESID)
         return bxm_launch_r<float, float, EPI_RESID>(a, R, py, pf, nt, lds, s);
     if (dtype == 4 && epi == EPI_RESID)
