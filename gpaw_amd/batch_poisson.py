@@ -170,7 +170,7 @@ class BatchedPoissonSolver:
             stream_handle()), 'batched transfer')
         return flat_out
 
-    def solve_mixed(self, phis, rhos, tol=1e-10, pre=2, post=2,
+    def solve_mixed(self, phis, rhos, tol=1e-10, pre=3, post=3,
                     coarse_sweeps=16, max_outer=100, graph=True):
         """Refine every phi_g (float64 device tensors, in/out) until
         |L phi_g - rho_g| / |rho_g| <= tol for all g.  Returns
