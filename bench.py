@@ -405,8 +405,9 @@ def stage_precond_128(dev, nbands=128, reps=5):
     """The FP32 kinetic preconditioner on one 128-band block of the
     1024-band workload (128^3 c128, first k-point): reps applications.
     Traffic model per fine point: complex128 residual read twice and result
-    written once (48 B), complex64 q0 written and read (16 B), coarse
-    levels ~1/7 of the fine work."""
+    written once (48 B; the fine level is one fused pass, its interpolated
+    d0' never stored), the coarse levels ~1/7 of the two-pass fine work
+    (64 B / 7)."""
     import torch
     from gpaw_amd import workloads
     from gpaw_amd.fd_operators import Laplace
@@ -425,7 +426,7 @@ def stage_precond_128(dev, nbands=128, reps=5):
         phase_cd = ph
     ms, _ = timed(lambda: P(R, K, out=out), reps, 1)
     pts = nbands * 128**3
-    gb = pts * 64 * (1 + 1 / 7) / 1e9
+    gb = pts * (48 + 64 / 7) / 1e9
     return {'ms': round(ms, 3), 'bands': nbands,
             'ms_per_1024_bands': round(ms * 1024 / nbands, 2),
             'GPts/s': round(pts / ms / 1e6, 1), 'model_GB': round(gb, 2),

@@ -21,7 +21,8 @@ SOURCES = ['rows_dma.hip', 'rows_dma32.hip', 'rows_f64.hip', 'rows_c128.hip', 'r
            'stencil.hip', 'batch.hip', 'transfer.hip', 'api.hip',
            'stencil_rows.hip', 'vector.hip', 'bands.hip', 'coarse.hip',
            'fftpoisson.hip', 'wstencil.hip', 'gs.hip',
-           'xfer_march.hip', 'wxmarch.hip', 'bxmarch.hip', 'bxfer_march.hip']
+           'xfer_march.hip', 'wxmarch.hip', 'bxmarch.hip', 'bxfer_march.hip',
+           'precond_fine.hip']
 
 FLAGS = [
     '-std=c++17', '-O3', '-fPIC', f'--offload-arch={ARCH}',

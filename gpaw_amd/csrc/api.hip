@@ -22,6 +22,7 @@
 #include "coarse.h"
 #include "gs.h"
 #include "wstencil.h"
+#include "precond_fine.h"
 #include "batch.h"
 
 namespace gfd {
@@ -780,6 +781,15 @@ static int fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         e.c0 = centre_coef(kin1);
         e.s = 4 * step;
         GFD_TRY(op_pass(kin1, dt, d1, q1, nin, ph, EPI_UPDATE, e, s, nullptr));
+    }
+    // both at once where the fine shape fits the fused kernel (complex
+    // FP32 chain): d0' is then never stored
+    if (fp32 && !real && kin0->use_fast()) {
+        const int st = launch_precond_fine(
+            kin0->ax, centre_coef(kin0), kin0->n, kin0->periodic,
+            intp1->plan, intp1->periodic, ph, q1, residuals, d0, nin, step,
+            s);
+        if (st != ENOTSUP_) return st;
     }
     // d0' = I1(-d1)                              (held in q0)
     {
@@ -1887,6 +1897,23 @@ extern "C" int gfd_set_tuning(const char* key, long value)
             return EINVAL_;
         }
         batch_set_kernel((int)value);
+        return OK;
+    }
+    if (std::strcmp(key, "precond_fine") == 0) {
+        // 0: the FP32 chain's fine level as two passes; 1: fused where it
+        // fits (default)
+        precond_fine_enable(value != 0);
+        return OK;
+    }
+    if (std::strcmp(key, "precond_fine_tile") == 0) {
+        // py | nys << 8 | xc << 16 (0: automatic)
+        const int py = (int)(value & 0xff), nys = (int)((value >> 8) & 0xff);
+        const int xc = (int)((value >> 16) & 0xffff);
+        if (value < 0 || py > 2 || nys > 256) {
+            set_error("gfd_set_tuning: precond_fine_tile fields out of range");
+            return EINVAL_;
+        }
+        precond_fine_force(py, nys, xc);
         return OK;
     }
     if (std::strcmp(key, "xfer_march_scale") == 0) {
