@@ -1903,6 +1903,7 @@ extern "C" int gfd_set_tuning(const char* key, long value)
         // 0: the FP32 chain's fine level as two passes; 1: fused where it
         // fits (default)
         precond_fine_enable(value != 0);
+        precond_fine_nt(value == 2);
         return OK;
     }
     if (std::strcmp(key, "precond_fine_tile") == 0) {

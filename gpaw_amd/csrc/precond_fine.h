@@ -20,6 +20,8 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
 
 // 0: off (always two passes), 1: on where the shape fits (default)
 void precond_fine_enable(bool on);
+// non-temporal stores of the result (default off)
+void precond_fine_nt(bool on);
 // force the tile (rows per thread py, thread rows nys, x chunk xc; 0:
 // automatic)
 void precond_fine_force(int py, int nys, int xc);

@@ -57,6 +57,7 @@ struct PfArgs {
     double c0, step;
     int tiles_y, chunks_x, xchunk, nblocks;
     long fgrid, cgrid;     // points per band
+    int nt_out;            // non-temporal result stores
 };
 
 // index inside [0, n) after one wrap; false outside a zero boundary
@@ -480,8 +481,19 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
                 const float2 d = fmac(v, ms, r);
                 o[k] = to_wide(-d);
             }
-            op[(long)j * n2] = o[0];
-            op[(long)j * n2 + 1] = o[1];
+            if (a.nt_out) {
+                // streamed result: non-temporal stores
+                typedef double nt2 __attribute__((ext_vector_type(2)));
+                __builtin_nontemporal_store(
+                    nt2{o[0].x, o[0].y},
+                    reinterpret_cast<nt2*>(op + (long)j * n2));
+                __builtin_nontemporal_store(
+                    nt2{o[1].x, o[1].y},
+                    reinterpret_cast<nt2*>(op + (long)j * n2 + 1));
+            } else {
+                op[(long)j * n2] = o[0];
+                op[(long)j * n2 + 1] = o[1];
+            }
         }
 #pragma unroll
         for (int s = 0; s + 1 < Q; s++)
@@ -497,9 +509,11 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
 // ------------------------------------------------------------------ host
 
 static bool g_pf_on = true;
-static int g_pf_py = 0, g_pf_nys = 0, g_pf_xc = 0;
+static int g_pf_py = 0, g_pf_nys = 0, g_pf_xc = 0, g_pf_nt = 0;
 
 void precond_fine_enable(bool on) { g_pf_on = on; }
+
+void precond_fine_nt(bool on) { g_pf_nt = on ? 1 : 0; }
 
 void precond_fine_force(int py, int nys, int xc)
 {
@@ -605,6 +619,7 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
             }
     a.c0 = c0;
     a.step = step;
+    a.nt_out = g_pf_nt;
     a.fgrid = (long)n0 * n1 * n2;
     a.cgrid = (long)ip.nin[0] * cplane;
     a.tiles_y = n1 / TY;

@@ -465,9 +465,10 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * "batch_xfer" (0 timed choice, 1 one output point per thread, 2 x-march
  * tiles, or a tile code nt | py << 12 | zmax << 16 | xc << 28 with nt >= 64
  * forcing that x-march tile) sets the batched transfers of batches tuned
- * afterwards; "precond_fine" (0/1, default 1) runs the fine level of the
- * complex FP32 preconditioner chain (gfd_fd_precond32) as one fused pass
- * where the shape fits (off: interpolation and update as two passes);
+ * afterwards; "precond_fine" (0/1/2, default 1) runs the fine level of
+ * the complex FP32 preconditioner chain (gfd_fd_precond32) as one fused
+ * pass where the shape fits (0: interpolation and update as two passes;
+ * 2: fused, with non-temporal stores of the result);
  * "precond_fine_tile" (py | nys << 8 | xc << 16, 0 automatic) forces its
  * tile: py rows per thread (1, 2), nys thread rows, xc x planes per block
  * (a forced tile that does not fit the grid makes the call fail).
