@@ -38,9 +38,9 @@ constexpr int PF_RP = 4;      // LDS columns each side of a row (even)
 constexpr int PF_MAXT = 512;  // threads per block (256 or 512 variants)
 
 struct PfArgs {
-    const float2* d1;
-    const double2* res;
-    double2* out;
+    const void* d1;        // T (c64 / f32)
+    const void* res;       // FP64 counterpart of T
+    void* out;
     int n[3];             // fine grid
     int m[3];             // coarse grid
     int off[3], skip0[3];  // interpolation plan
@@ -60,6 +60,39 @@ struct PfArgs {
     int nt_out;            // non-temporal result stores
 };
 
+// Data type T (float2: complex, float: real), its FP64 counterpart W and
+// the vector P of a lane's two z points
+template <typename T> struct PfT;
+template <> struct PfT<float2> {
+    typedef float4 P;
+    typedef double2 W;
+    static constexpr bool cplx = true;
+};
+template <> struct PfT<float> {
+    typedef float2 P;
+    typedef double W;
+    static constexpr bool cplx = false;
+};
+__device__ inline float4 pf_pack(float2 a, float2 b)
+{
+    return make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ inline float2 pf_pack(float a, float b) { return make_float2(a, b); }
+__device__ inline void pf_unpack(float4 u, float2& a, float2& b)
+{
+    a = make_float2(u.x, u.y);
+    b = make_float2(u.z, u.w);
+}
+__device__ inline void pf_unpack(float2 u, float& a, float& b)
+{
+    a = u.x;
+    b = u.y;
+}
+
+// Bloch factor times a value (real data: the factor is 1)
+__device__ inline float2 pf_cmul(float2 f, float2 v) { return cmul(f, v); }
+__device__ inline float pf_cmul(float2, float v) { return v; }
+
 // index inside [0, n) after one wrap; false outside a zero boundary
 __device__ inline bool pf_wrap(int& i, int n, int periodic)
 {
@@ -69,11 +102,14 @@ __device__ inline bool pf_wrap(int& i, int n, int periodic)
     return true;
 }
 
-template <int R, int PY, int HM, int MAXT>
+template <typename T, int R, int PY, int HM, int MAXT>
 __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
 {
+    typedef typename PfT<T>::P P;
+    typedef typename PfT<T>::W W;
+    constexpr bool CPLX = PfT<T>::cplx;
     extern __shared__ __align__(16) unsigned char pf_smem[];
-    float2* lds = reinterpret_cast<float2*>(pf_smem);
+    T* lds = reinterpret_cast<T*>(pf_smem);
     constexpr int Q = 2 * R + 1;
 
     const int L = xcd_remap(blockIdx.x, a.nblocks);
@@ -90,12 +126,14 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
     const int BUF = (TY + 2 * R) * LW;
     const int y0 = yt * TY;
     const int xs = xt * a.xchunk, xe = min(xs + a.xchunk, n0);
-    const float2* __restrict__ gc = a.d1 + b * a.cgrid;
+    const T* __restrict__ gc = (const T*)a.d1 + b * a.cgrid;
     const long cplane = (long)m1 * m2;
-    const unsigned cpb = (unsigned)(cplane * 8);
+    const unsigned cpb = (unsigned)(cplane * (long)sizeof(T));
     const bool zodd = (a.skip0[2] & 1) != 0;
-    float2* ctab = lds + 2 * BUF;  // the coarse factors
-    if (tid < 27) ctab[tid] = a.cf[tid];
+    // the coarse Bloch factors (complex data; 2 BUF is even, so the table
+    // is 8-byte aligned)
+    float2* ctab = reinterpret_cast<float2*>(lds + 2 * BUF);
+    if (CPLX && tid < 27) ctab[tid] = a.cf[tid];
     __syncthreads();
 
     // coarse elements of a point (fine row yf inside the grid, z pair t):
@@ -120,7 +158,8 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
                                 pf_wrap(row, m1, a.cper[1]) &&
                                 pf_wrap(col, m2, a.cper[2]);
                 o[2 * r + k] =
-                    ok ? (unsigned)(row * m2 + col) * 8u : BXM_OOB;
+                    ok ? (unsigned)(row * m2 + col) * (unsigned)sizeof(T)
+                       : BXM_OOB;
                 codes |= (unsigned)(ys + 3 * zs) << (4 * (2 * r + k));
             }
     };
@@ -133,33 +172,33 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         return brsrc(gc + (ok ? (long)cw * cplane : 0L), ok ? cpb : 0u);
     };
     auto raw = [&](const __amdgpu_buffer_rsrc_t r, const unsigned (&o)[4],
-                   float2 (&v)[4]) {
+                   T (&v)[4]) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) v[e] = bload<float2>(r, o[e]);
+        for (int e = 0; e < 4; e++) v[e] = bload<T>(r, o[e]);
     };
     // z pass of both coarse rows of a point from its raw values (Bloch
     // factors first): z[r][k] at fine z 2t + k
-    auto zrows = [&](float2 (&v)[4], unsigned codes, int xside,
-                     float2 (&z)[2][2]) {
-        if (a.phased) {
+    auto zrows = [&](T (&v)[4], unsigned codes, int xside,
+                     T (&z)[2][2]) {
+        if (CPLX && a.phased) {
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const unsigned t = 9u * (unsigned)xside +
                                    ((codes >> (4 * e)) & 15u);
-                if (t) v[e] = cmul(ctab[t], v[e]);
+                if (t) v[e] = pf_cmul(ctab[t], v[e]);
             }
         }
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const float2 h = 0.5f * (v[2 * r] + v[2 * r + 1]);
+            const T h = 0.5f * (v[2 * r] + v[2 * r + 1]);
             z[r][0] = zodd ? h : v[2 * r];
             z[r][1] = zodd ? v[2 * r + 1] : h;
         }
     };
     // Y_c of a point (y pass: one coarse row, or the mean of two)
-    auto yz = [&](float2 (&v)[4], unsigned codes, int xside, bool two_y,
-                  float2 (&Y)[2]) {
-        float2 z[2][2];
+    auto yz = [&](T (&v)[4], unsigned codes, int xside, bool two_y,
+                  T (&Y)[2]) {
+        T z[2][2];
         zrows(v, codes, xside, z);
 #pragma unroll
         for (int k = 0; k < 2; k++)
@@ -178,13 +217,13 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
     for (int g = 0; g < NG; g++)
         setup(y0 + sg * PY + g, tx, oo[g], ocode[g], oty[g], PY == 2);
     // Y of the own rows from the raw values of their groups
-    auto own_y = [&](float2 (&v)[NG][4], int xside, float2 (&Y)[PY][2]) {
+    auto own_y = [&](T (&v)[NG][4], int xside, T (&Y)[PY][2]) {
         if constexpr (PY == 2) {
-            float2 z[2][2];
+            T z[2][2];
             zrows(v[0], ocode[0], xside, z);
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                const float2 mid = 0.5f * (z[0][k] + z[1][k]);
+                const T mid = 0.5f * (z[0][k] + z[1][k]);
                 Y[0][k] = oty[0] ? mid : z[0][k];
                 Y[1][k] = oty[0] ? z[1][k] : mid;
             }
@@ -221,12 +260,12 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
 
     // incremental coarse state of each point set: Y of planes c, c + 1
     // and the raw values of c + 2 in flight
-    float2 oA[PY][2], oB[PY][2], oN[NG][4];
-    float2 hA[HM][2], hB[HM][2], hN[HM][4];
+    T oA[PY][2], oB[PY][2], oN[NG][4];
+    T hA[HM][2], hB[HM][2], hN[HM][4];
     int oc = -0x40000000, hc0 = -0x40000000, oNx = 0, hNx = 0;
 
     // d0' of fine plane p (inside the grid) at the own points
-    auto own_plane = [&](int p, float2 (&dst)[PY][2]) {
+    auto own_plane = [&](int p, T (&dst)[PY][2]) {
         const int qx = p + a.skip0[0];
         const int c = (qx >> 1) + a.off[0];
         if (c == oc + 1) {
@@ -240,7 +279,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             int xa, xb;
             const __amdgpu_buffer_rsrc_t ra = cres(c, xa);
             const __amdgpu_buffer_rsrc_t rb = cres(c + 1, xb);
-            float2 va[NG][4], vb[NG][4];
+            T va[NG][4], vb[NG][4];
 #pragma unroll
             for (int g = 0; g < NG; g++) {
                 raw(ra, oo[g], va[g]);
@@ -264,13 +303,13 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
     };
     // any fine plane: wrapped with its Bloch factor, zero outside a zero
     // boundary
-    auto own_any = [&](int p, float2 (&dst)[PY][2]) {
+    auto own_any = [&](int p, T (&dst)[PY][2]) {
         int pw = p;
         const int side = p < 0 ? 1 : (p >= n0 ? 2 : 0);
         if (!pf_wrap(pw, n0, a.fper[0])) {
 #pragma unroll
             for (int j = 0; j < PY; j++)
-                dst[j][0] = dst[j][1] = make_float2(0.f, 0.f);
+                dst[j][0] = dst[j][1] = zero_of<T>();
             return;
         }
         own_plane(pw, dst);
@@ -279,12 +318,12 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
 #pragma unroll
             for (int j = 0; j < PY; j++)
 #pragma unroll
-                for (int k = 0; k < 2; k++) dst[j][k] = cmul(f, dst[j][k]);
+                for (int k = 0; k < 2; k++) dst[j][k] = pf_cmul(f, dst[j][k]);
         }
     };
     // d0' of fine plane p (inside the grid) at the halo slots, with the
     // fine y factor
-    auto halo_plane = [&](int p, float2 (&dst)[HM][2]) {
+    auto halo_plane = [&](int p, T (&dst)[HM][2]) {
         const int qx = p + a.skip0[0];
         const int c = (qx >> 1) + a.off[0];
         if (c == hc0 + 1) {
@@ -300,7 +339,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             const __amdgpu_buffer_rsrc_t rb = cres(c + 1, xb);
 #pragma unroll
             for (int m = 0; m < HM; m++) {
-                float2 va[4], vb[4];
+                T va[4], vb[4];
                 raw(ra, ho[m], va);
                 raw(rb, ho[m], vb);
                 yz(va, hcode[m], xa, hty[m], hA[m]);
@@ -318,10 +357,10 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         for (int m = 0; m < HM; m++) {
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                float2 v = -(odd ? 0.5f * (hA[m][k] + hB[m][k]) : hA[m][k]);
-                if (hside[m] == 3) v = make_float2(0.f, 0.f);
+                T v = -(odd ? 0.5f * (hA[m][k] + hB[m][k]) : hA[m][k]);
+                if (hside[m] == 3) v = zero_of<T>();
                 if (hside[m] == 1 || hside[m] == 2)
-                    v = cmul(a.ff[1][hside[m] == 1 ? 1 : 0], v);
+                    v = pf_cmul(a.ff[1][hside[m] == 1 ? 1 : 0], v);
                 dst[m][k] = v;
             }
         }
@@ -350,21 +389,21 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         for (int j = 0; j < PY; j++)
 #pragma unroll
             for (int bb = 0; bb < 2; bb++) {
-                float2* row = lds + bb * BUF + (R + sg * PY + j) * LW;
+                T* row = lds + bb * BUF + (R + sg * PY + j) * LW;
                 const int cc = zhi ? col + n2 : col - n2;
-                row[cc] = make_float2(0.f, 0.f);
-                row[cc + 1] = make_float2(0.f, 0.f);
+                row[cc] = zero_of<T>();
+                row[cc + 1] = zero_of<T>();
             }
     }
 
-    float2 q[Q][PY][2];
+    T q[Q][PY][2];
 #pragma unroll
     for (int s = 0; s < Q; s++) own_any(xs - R + s, q[s]);
 
     const long fplane = (long)n1 * n2;
     const long obase = b * a.fgrid + (long)(y0 + sg * PY) * n2 + 2 * tx;
-    auto ld_res = [&](int x, double2 (&rs)[PY][2]) {
-        const double2* rp = a.res + obase + (long)x * fplane;
+    auto ld_res = [&](int x, W (&rs)[PY][2]) {
+        const W* rp = (const W*)a.res + obase + (long)x * fplane;
 #pragma unroll
         for (int j = 0; j < PY; j++) {
             rs[j][0] = rp[(long)j * n2];
@@ -372,58 +411,55 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         }
     };
     for (int x = xs; x < xe; x++) {
-        float2* buf = lds + ((x - xs) & 1) * BUF;
+        T* buf = lds + ((x - xs) & 1) * BUF;
         // the residual of this plane: in flight over the halo, the LDS
         // stores and the barrier
-        double2 rc[PY][2];
+        W rc[PY][2];
         ld_res(x, rc);
-        float2 hv[HM][2];
+        T hv[HM][2];
         halo_plane(x, hv);
 #pragma unroll
         for (int j = 0; j < PY; j++) {
-            float2* row = buf + (R + sg * PY + j) * LW;
-            *reinterpret_cast<float4*>(row + col) =
-                make_float4(q[R][j][0].x, q[R][j][0].y, q[R][j][1].x,
-                            q[R][j][1].y);
+            T* row = buf + (R + sg * PY + j) * LW;
+            *reinterpret_cast<P*>(row + col) =
+                pf_pack(q[R][j][0], q[R][j][1]);
             if (zmir && zhi) {
-                const float2 u0 = cmul(a.ff[2][0], q[R][j][0]);
-                const float2 u1 = cmul(a.ff[2][0], q[R][j][1]);
-                *reinterpret_cast<float4*>(row + col + n2) =
-                    make_float4(u0.x, u0.y, u1.x, u1.y);
+                const T u0 = pf_cmul(a.ff[2][0], q[R][j][0]);
+                const T u1 = pf_cmul(a.ff[2][0], q[R][j][1]);
+                *reinterpret_cast<P*>(row + col + n2) =
+                    pf_pack(u0, u1);
             }
             if (zmir && zlo) {
-                const float2 u0 = cmul(a.ff[2][1], q[R][j][0]);
-                const float2 u1 = cmul(a.ff[2][1], q[R][j][1]);
-                *reinterpret_cast<float4*>(row + col - n2) =
-                    make_float4(u0.x, u0.y, u1.x, u1.y);
+                const T u0 = pf_cmul(a.ff[2][1], q[R][j][0]);
+                const T u1 = pf_cmul(a.ff[2][1], q[R][j][1]);
+                *reinterpret_cast<P*>(row + col - n2) =
+                    pf_pack(u0, u1);
             }
         }
 #pragma unroll
         for (int m = 0; m < HM; m++)
             if (hl[m] >= 0)
-                *reinterpret_cast<float4*>(buf + hl[m]) =
-                    make_float4(hv[m][0].x, hv[m][0].y, hv[m][1].x,
-                                hv[m][1].y);
+                *reinterpret_cast<P*>(buf + hl[m]) =
+                    pf_pack(hv[m][0], hv[m][1]);
         __syncthreads();
 
-        double2* op = a.out + obase + (long)x * fplane;
+        W* op = (W*)a.out + obase + (long)x * fplane;
 #pragma unroll
         for (int j = 0; j < PY; j++) {
             const int lrow = (R + sg * PY + j) * LW + col;
-            float2 acc[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+            T acc[2] = {zero_of<T>(), zero_of<T>()};
             if (a.has[2]) {
                 // z - 4 .. z + 5 around the pair
-                float2 w[10];
+                T w[10];
 #pragma unroll
                 for (int c = 0; c < 5; c++) {
                     if (c == 2) {
                         w[4] = q[R][j][0];
                         w[5] = q[R][j][1];
                     } else {
-                        const float4 u = *reinterpret_cast<const float4*>(
+                        const P u = *reinterpret_cast<const P*>(
                             buf + lrow + 2 * (c - 2));
-                        w[2 * c] = make_float2(u.x, u.y);
-                        w[2 * c + 1] = make_float2(u.z, u.w);
+                        pf_unpack(u, w[2 * c], w[2 * c + 1]);
                     }
                 }
 #pragma unroll
@@ -437,24 +473,22 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             if (a.has[1]) {
 #pragma unroll
                 for (int kk = 1; kk <= R; kk++) {
-                    float2 up[2], dn[2];
+                    T up[2], dn[2];
                     if (j + kk < PY) {
                         up[0] = q[R][j + kk][0];
                         up[1] = q[R][j + kk][1];
                     } else {
-                        const float4 u = *reinterpret_cast<const float4*>(
+                        const P u = *reinterpret_cast<const P*>(
                             buf + lrow + kk * LW);
-                        up[0] = make_float2(u.x, u.y);
-                        up[1] = make_float2(u.z, u.w);
+                        pf_unpack(u, up[0], up[1]);
                     }
                     if (j - kk >= 0) {
                         dn[0] = q[R][j - kk][0];
                         dn[1] = q[R][j - kk][1];
                     } else {
-                        const float4 u = *reinterpret_cast<const float4*>(
+                        const P u = *reinterpret_cast<const P*>(
                             buf + lrow - kk * LW);
-                        dn[0] = make_float2(u.x, u.y);
-                        dn[1] = make_float2(u.z, u.w);
+                        pf_unpack(u, dn[0], dn[1]);
                     }
 #pragma unroll
                     for (int k = 0; k < 2; k++) {
@@ -472,24 +506,30 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
                         acc[k] = fmac(acc[k], cmx[kk - 1], q[R - kk][j][k]);
                     }
             }
-            double2 o[2];
+            W o[2];
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                const float2 v = q[R][j][k];
-                float2 r = fmac(acc[k], c0, v);
+                const T v = q[R][j][k];
+                T r = fmac(acc[k], c0, v);
                 r = r - to_narrow(rc[j][k]);
-                const float2 d = fmac(v, ms, r);
+                const T d = fmac(v, ms, r);
                 o[k] = to_wide(-d);
             }
             if (a.nt_out) {
                 // streamed result: non-temporal stores
                 typedef double nt2 __attribute__((ext_vector_type(2)));
-                __builtin_nontemporal_store(
-                    nt2{o[0].x, o[0].y},
-                    reinterpret_cast<nt2*>(op + (long)j * n2));
-                __builtin_nontemporal_store(
-                    nt2{o[1].x, o[1].y},
-                    reinterpret_cast<nt2*>(op + (long)j * n2 + 1));
+                if constexpr (CPLX) {
+                    __builtin_nontemporal_store(
+                        nt2{o[0].x, o[0].y},
+                        reinterpret_cast<nt2*>(op + (long)j * n2));
+                    __builtin_nontemporal_store(
+                        nt2{o[1].x, o[1].y},
+                        reinterpret_cast<nt2*>(op + (long)j * n2 + 1));
+                } else {
+                    __builtin_nontemporal_store(
+                        nt2{o[0], o[1]},
+                        reinterpret_cast<nt2*>(op + (long)j * n2));
+                }
             } else {
                 op[(long)j * n2] = o[0];
                 op[(long)j * n2 + 1] = o[1];
@@ -522,25 +562,42 @@ void precond_fine_force(int py, int nys, int xc)
     g_pf_xc = xc;
 }
 
-template <int R, int PY, int HM>
+template <typename T, int R, int PY, int HM>
 static void pf_launch(const PfArgs& a, int NZ, int NYS, size_t lds,
                       hipStream_t s)
 {
     if (NZ * NYS > 256)
-        hipLaunchKernelGGL((k_pfine<R, PY, HM, 512>),
+        hipLaunchKernelGGL((k_pfine<T, R, PY, HM, 512>),
                            dim3((unsigned)a.nblocks), dim3(NZ, NYS), lds, s,
                            a);
     else
-        hipLaunchKernelGGL((k_pfine<R, PY, HM, 256>),
+        hipLaunchKernelGGL((k_pfine<T, R, PY, HM, 256>),
                            dim3((unsigned)a.nblocks), dim3(NZ, NYS), lds, s,
                            a);
+}
+
+template <typename T>
+static int pf_dispatch(const PfArgs& a, int R, int PY, int HM, int NZ,
+                       int NYS, size_t lds, hipStream_t s)
+{
+#define GFD_PF(RR, PP, HH)                                                     \
+    if (R == RR && PY == PP && HM == HH) {                                     \
+        pf_launch<T, RR, PP, HH>(a, NZ, NYS, lds, s);                          \
+        GFD_LAUNCH_CHECK();                                                    \
+        return OK;                                                             \
+    }
+    GFD_PF(1, 1, 1) GFD_PF(1, 1, 2) GFD_PF(1, 2, 1) GFD_PF(1, 2, 2)
+    GFD_PF(2, 1, 1) GFD_PF(2, 1, 2) GFD_PF(2, 2, 1) GFD_PF(2, 2, 2)
+    GFD_PF(3, 1, 1) GFD_PF(3, 1, 2) GFD_PF(3, 2, 1) GFD_PF(3, 2, 2)
+#undef GFD_PF
+    return ENOTSUP_;
 }
 
 int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                         const int fine_periodic[3], const XferPlan& ip,
                         const int coarse_periodic[3], const double* phases,
                         const void* d1, const void* res, void* out, long nin,
-                        double step, hipStream_t s)
+                        double step, int real, hipStream_t s)
 {
     if (!g_pf_on || !ip.interpolate || ip.K != 2 || ax.R < 1 || ax.R > 3)
         return ENOTSUP_;
@@ -576,15 +633,18 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                   g_pf_py, g_pf_nys, n0, n1, n2);
         return EINVAL_;
     }
-    const size_t lds = (2 * (size_t)(TY + 2 * R) * (n2 + 2 * PF_RP) + 28) * 8;
+    // two plane buffers of T, then the 27 complex factors (8-byte aligned)
+    const size_t es = real ? 4 : 8;
+    const size_t lds =
+        2 * (size_t)(TY + 2 * R) * (n2 + 2 * PF_RP) * es + 28 * 8;
     if (lds > 64 * 1024) return ENOTSUP_;
     const long cplane = (long)ip.nin[1] * ip.nin[2];
-    if (cplane * 8 >= 0x7fffffffL) return ENOTSUP_;
+    if (cplane * (long)es >= 0x7fffffffL) return ENOTSUP_;
 
     PfArgs a;
-    a.d1 = (const float2*)d1;
-    a.res = (const double2*)res;
-    a.out = (double2*)out;
+    a.d1 = d1;
+    a.res = res;
+    a.out = out;
     a.phased = 0;
     double2 ph[3][2];
     for (int c = 0; c < 3; c++) {
@@ -634,17 +694,8 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
     if (nb > 0x7fffffffL) return ENOTSUP_;
     a.nblocks = (int)nb;
 
-#define GFD_PF(RR, PP, HH)                                                     \
-    if (R == RR && PY == PP && HM == HH) {                                     \
-        pf_launch<RR, PP, HH>(a, NZ, NYS, lds, s);                             \
-        GFD_LAUNCH_CHECK();                                                    \
-        return OK;                                                             \
-    }
-    GFD_PF(1, 1, 1) GFD_PF(1, 1, 2) GFD_PF(1, 2, 1) GFD_PF(1, 2, 2)
-    GFD_PF(2, 1, 1) GFD_PF(2, 1, 2) GFD_PF(2, 2, 1) GFD_PF(2, 2, 2)
-    GFD_PF(3, 1, 1) GFD_PF(3, 1, 2) GFD_PF(3, 2, 1) GFD_PF(3, 2, 2)
-#undef GFD_PF
-    return ENOTSUP_;
+    return real ? pf_dispatch<float>(a, R, PY, HM, NZ, NYS, lds, s)
+                : pf_dispatch<float2>(a, R, PY, HM, NZ, NYS, lds, s);
 }
 
 }  // namespace gfd
