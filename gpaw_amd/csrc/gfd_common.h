@@ -163,6 +163,34 @@ struct BC {
     double2 phase[3][2];  // [axis][0]: above the upper end, [1]: below lower
 };
 
+// Flat point index gid of a batch of n0 x n1 x n2 grids (grid points each,
+// total points in the launch) -> (item b, i0, i1, i2).  32-bit divisions
+// when the launch has fewer than 2^31 points (a 64-bit division expands to
+// about a hundred instructions, more than the point's own work in the
+// one-point-per-thread kernels); 64-bit beyond.
+__device__ inline void flat_split(long gid, long total, long grid, int n1,
+                                  int n2, long& b, int& i0, int& i1, int& i2)
+{
+    if (total <= 0x7fffffffL) {
+        const unsigned g = (unsigned)gid, gs = (unsigned)grid;
+        const unsigned bb = g / gs;
+        const unsigned r = g - bb * gs;
+        const unsigned q = r / (unsigned)n2;
+        i2 = (int)(r - q * (unsigned)n2);
+        const unsigned q1 = q / (unsigned)n1;
+        i1 = (int)(q - q1 * (unsigned)n1);
+        i0 = (int)q1;
+        b = bb;
+    } else {
+        b = gid / grid;
+        long r = gid - b * grid;
+        i2 = (int)(r % n2);
+        r /= n2;
+        i1 = (int)(r % n1);
+        i0 = (int)(r / n1);
+    }
+}
+
 // Map a possibly out-of-range index along one axis to (index, factor).
 // Returns false when the value is zero (zero boundary).  `i` must lie in
 // [-n, 2n) -- the stencil range never exceeds the grid size.

@@ -312,12 +312,9 @@ __global__ void __launch_bounds__(GEN_BLOCK) k_general(GenArgs<T> a)
     const long gid = (long)blockIdx.x * GEN_BLOCK + threadIdx.x;
     double nrm = 0.0;
     if (gid < a.total) {
-        const long b = gid / a.ng;
-        long p = gid - b * a.ng;
-        const int i2 = (int)(p % a.n2);
-        p /= a.n2;
-        const int i1 = (int)(p % a.n1);
-        const int i0 = (int)(p / a.n1);
+        long b;
+        int i0, i1, i2;
+        flat_split(gid, a.total, a.ng, a.n1, a.n2, b, i0, i1, i2);
         const T* __restrict__ g = a.in + b * a.ng;
         T acc = zero_of<T>();
         for (int c = 0; c < a.P; c++) {

@@ -106,12 +106,9 @@ __global__ void __launch_bounds__(256) k_xfer1d(XArgs<T> a)
     typedef typename ScalarOf<T>::type S;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     if (gid >= a.total) return;
-    const long b = gid / a.out_grid;
-    long r = gid - b * a.out_grid;
-    const int o2 = (int)(r % a.d[2]);
-    r /= a.d[2];
-    const int o1 = (int)(r % a.d[1]);
-    const int o0 = (int)(r / a.d[1]);
+    long b;
+    int o0, o1, o2;
+    flat_split(gid, a.total, a.out_grid, a.d[1], a.d[2], b, o0, o1, o2);
     // input position with the pass axis at 0
     long pos;
     int i;
@@ -242,12 +239,9 @@ __global__ void __launch_bounds__(256) k_restrict2(F3Args<T, TI> a)
     typedef typename ScalarOf<T>::type S;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     if (gid >= a.total) return;
-    const long b = gid / a.out_grid;
-    long r = gid - b * a.out_grid;
-    const int o2 = (int)(r % a.no[2]);
-    r /= a.no[2];
-    const int o1 = (int)(r % a.no[1]);
-    const int o0 = (int)(r / a.no[1]);
+    long b;
+    int o0, o1, o2;
+    flat_split(gid, a.total, a.out_grid, a.no[1], a.no[2], b, o0, o1, o2);
     const int c0 = 2 * o0 + a.off[0], c1 = 2 * o1 + a.off[1],
               c2 = 2 * o2 + a.off[2];
     const TI* __restrict__ g = a.in + b * a.in_grid;
@@ -275,12 +269,9 @@ __global__ void __launch_bounds__(256) k_interp2(F3Args<T, TI> a)
     typedef typename ScalarOf<T>::type S;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     if (gid >= a.total) return;
-    const long b = gid / a.out_grid;
-    long r = gid - b * a.out_grid;
-    const int o2 = (int)(r % a.no[2]);
-    r /= a.no[2];
-    const int o1 = (int)(r % a.no[1]);
-    const int o0 = (int)(r / a.no[1]);
+    long b;
+    int o0, o1, o2;
+    flat_split(gid, a.total, a.out_grid, a.no[1], a.no[2], b, o0, o1, o2);
     const int q0 = o0 + a.skip0[0], q1 = o1 + a.skip0[1], q2 = o2 + a.skip0[2];
     const int c0 = (q0 >> 1) + a.off[0], c1 = (q1 >> 1) + a.off[1],
               c2 = (q2 >> 1) + a.off[2];
@@ -343,12 +334,9 @@ __global__ void __launch_bounds__(256) k_interp2p(F3Args<T, TI> a)
     const long gp = (long)blockIdx.x * 256 + threadIdx.x;
     if (2 * gp >= a.total) return;
     const long gid = 2 * gp;
-    const long b = gid / a.out_grid;
-    long r = gid - b * a.out_grid;
-    const int o2 = (int)(r % a.no[2]);
-    r /= a.no[2];
-    const int o1 = (int)(r % a.no[1]);
-    const int o0 = (int)(r / a.no[1]);
+    long b;
+    int o0, o1, o2;
+    flat_split(gid, a.total, a.out_grid, a.no[1], a.no[2], b, o0, o1, o2);
     const TI* __restrict__ g = a.in + b * a.in_grid;
     const T v0 = interp2_point(a, g, o0, o1, o2);
     const T v1 = interp2_point(a, g, o0, o1, o2 + 1);
