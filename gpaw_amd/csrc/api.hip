@@ -632,7 +632,9 @@ static size_t xfer_scratch_bytes(const gfd_transformer* t, int dtype,
                                  long nin)
 {
     const size_t es = dtype == 1 ? 16 : (dtype == 2 ? 4 : 8);  // 3: c64
-    return t->plan.K == 2 ? 0 : es * xfer_scratch_elems(t->plan, nin) + 16;
+    return t->plan.K == 2 && !xfer_passes_forced()
+               ? 0
+               : es * xfer_scratch_elems(t->plan, nin) + 16;
 }
 
 // Time the transfer kernel families for this shape once (not under graph
@@ -660,7 +662,7 @@ static int xfer_dt(gfd_transformer* t, int dtype, const void* in, void* out,
                    long nin, const double* phases, const XEpiArgs& ea,
                    hipStream_t s, void* work = nullptr)
 {
-    if (!work && t->plan.K != 2) {
+    if (!work && (t->plan.K != 2 || xfer_passes_forced())) {
         GFD_TRY(t->scratch.ensure(xfer_scratch_bytes(t, dtype, nin), s));
         work = t->scratch.p;
     }

@@ -45,6 +45,9 @@ void xfer_force_passes(bool on);
 // Use the x-marching fused kernels (every K; default on).  Off: K = 2 by
 // the one-point-per-thread fused kernels, K > 2 by three 1-D passes.
 void xfer_set_march(bool on);
+// Whether K = 2 transfers are forced onto the separable passes (they then
+// need the passes' work space).
+bool xfer_passes_forced();
 // Tiles per resident block of the marching kernels, in units of four
 // (default 1: about four).
 void xfer_set_march_scale(int s);

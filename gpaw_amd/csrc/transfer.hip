@@ -509,6 +509,8 @@ static int run_transfer_k(const XferPlan& p, const BC& bc, const void* in,
 static bool g_xfer_force_passes = false;
 static bool g_xfer_march = true;
 
+bool xfer_passes_forced() { return g_xfer_force_passes; }
+
 void xfer_force_passes(bool on) { g_xfer_force_passes = on; }
 void xfer_set_march(bool on) { g_xfer_march = on; }
 
@@ -647,7 +649,10 @@ static int launch_other(const XferPlan& p, int dtype, const BC& bc,
         set_error("transfer: complex64 data only for K = 2");
         return ENOTSUP_;
     }
-    if (p.K == 2 && (!g_xfer_force_passes || dtype == 3 || ea.in_wide)) {
+    // K = 2 on the separable passes only when forced and given their work
+    // space (caller-owned work spaces are sized for the fused kernel)
+    if (p.K == 2 &&
+        (!g_xfer_force_passes || dtype == 3 || ea.in_wide || !scratch)) {
         if (dtype == 0)
             return launch_fused2<double>(p, bc, in, out, nin, ea, stream);
         if (dtype == 1)

@@ -90,3 +90,37 @@ def test_reference_gpu_case(pbc, nn):
     t = Transformer(coarse, gd, nn, dtype)
     assert np.abs(gpu_transfer(t, ac_ref, phase) -
                   ref_transfer(t, ac_ref, phase)).max() < 1e-14
+
+
+@pytest.mark.parametrize('nn', [1, 2, 3, 4])
+def test_forced_passes(nn):
+    """Every K on the three separable passes (forced; K = 2 then needs the
+    passes' work space, which the transformer allocates while the passes
+    are forced) against the oracle: restriction and interpolation, real and
+    complex with Bloch phases, periodic, zero and mixed boundaries."""
+    from gpaw_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(300 + nn)
+    for N in [(16, 12, 20), (24, 40, 32)]:
+        for pbc in [True, False, (False, True, True)]:
+            fine = GridDescriptor(N, np.array(N) * 0.2, pbc)
+            coarse = fine.coarsen()
+            for interp in (False, True):
+                gin, gout = (coarse, fine) if interp else (fine, coarse)
+                for dtype in (float, complex):
+                    try:
+                        t = Transformer(gin, gout, nn, dtype)
+                    except ValueError:
+                        continue
+                    a = rand(rng, (3,) + tuple(gin.n_c), dtype)
+                    ph = (bloch_phases(rng, gin.pbc_c)
+                          if dtype == complex else None)
+                    say('passes', nn, N, pbc, interp, dtype.__name__)
+                    try:
+                        lib.gfd_set_tuning(b'xfer_march', 0)
+                        lib.gfd_set_tuning(b'xfer_passes', 1)
+                        got = gpu_transfer(t, a, ph)
+                    finally:
+                        lib.gfd_set_tuning(b'xfer_passes', 0)
+                        lib.gfd_set_tuning(b'xfer_march', 1)
+                    assert relerr(got, ref_transfer(t, a, ph)) < TOL
