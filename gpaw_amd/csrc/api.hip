@@ -784,13 +784,13 @@ static int fd_precond(gfd_transformer* rest0, gfd_transformer* rest1,
         e.s = 4 * step;
         GFD_TRY(op_pass(kin1, dt, d1, q1, nin, ph, EPI_UPDATE, e, s, nullptr));
     }
-    // both at once where the fine shape fits the fused kernel (FP32
-    // chain): d0' is then never stored
-    if (fp32 && kin0->use_fast()) {
+    // both at once where the fine shape fits the fused kernel: d0' is then
+    // never stored
+    if (kin0->use_fast()) {
         const int st = launch_precond_fine(
             kin0->ax, centre_coef(kin0), kin0->n, kin0->periodic,
             intp1->plan, intp1->periodic, ph, q1, residuals, d0, nin, step,
-            real, s);
+            real, fp32 ? 0 : 1, s);
         if (st != ENOTSUP_) return st;
     }
     // d0' = I1(-d1)                              (held in q0)

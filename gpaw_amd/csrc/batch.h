@@ -107,9 +107,13 @@ __device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff)
     if constexpr (sizeof(T) == 4) {
         return __builtin_bit_cast(
             T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
-    } else {
+    } else if constexpr (sizeof(T) == 8) {
         return __builtin_bit_cast(
             T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+    } else {
+        static_assert(sizeof(T) == 16, "4, 8 or 16-byte elements");
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
     }
 }
 

@@ -1,6 +1,7 @@
-// Fused fine level of the FP32 kinetic preconditioner (real or complex): the
-// interpolation d0' = I1(-d1) and the update d0 = -(d0' - step (K0 d0' -
-// res)) in one pass, d0' never written to memory.
+// Fused fine level of the kinetic preconditioner (real or complex; the
+// FP32 chain or the FP64 one): the interpolation d0' = I1(-d1) and the
+// update d0 = -(d0' - step (K0 d0' - res)) in one pass, d0' never written
+// to memory.
 #pragma once
 
 #include "stencil.h"
@@ -8,15 +9,16 @@
 
 namespace gfd {
 
-// Launch the fused pass on nin bands: d1 (f32 / c64, the interpolation's
-// coarse input, plan ip), res and out (f64 / c128 on the fine grid n of the axis
+// Launch the fused pass on nin bands: d1 (f32 / c64, or f64 / c128 with
+// fp64, the interpolation's coarse input, plan ip), res and out (f64 / c128
+// on the fine grid n of the axis
 // stencil ax).  ENOTSUP_ when the shape or stencil does not fit the kernel
 // (the caller then runs the two passes); OK when launched.
 int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                         const int fine_periodic[3], const XferPlan& ip,
                         const int coarse_periodic[3], const double* phases,
                         const void* d1, const void* res, void* out, long nin,
-                        double step, int real, hipStream_t s);
+                        double step, int real, int fp64, hipStream_t s);
 
 // 0: off (always two passes), 1: on where the shape fits (default)
 void precond_fine_enable(bool on);

@@ -29,6 +29,7 @@
 #include "precond_fine.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #include "batch.h"
 #include "stencil_dev.h"
@@ -46,12 +47,14 @@ struct PfArgs {
     int m[3];             // coarse grid
     int off[3], skip0[3];  // interpolation plan
     int fper[3], cper[3];  // periodic axes, fine / coarse
-    // Bloch factors in FP32: coarse element of sides (x, y, z) at
-    // cf[9 x + y + 3 z] (the product x, then y, then z in FP64, rounded
-    // once, as the marching transfer kernel forms it; 1 below, 2 above);
-    // fine axis c at ff[c][0] above, [1] below
+    // Bloch factors: coarse element of sides (x, y, z) at cf[9 x + y + 3 z]
+    // (the product x, then y, then z in FP64, as the marching transfer
+    // kernel forms it; 1 below, 2 above), fine axis c at ff[c][0] above,
+    // [1] below; rounded to FP32 for FP32 data (cf, ff), FP64 (cfd, ffd)
     float2 cf[27];
     float2 ff[3][2];
+    double2 cfd[27];
+    double2 ffd[3][2];
     int phased;            // some factor differs from 1
     double cp[3][3], cm[3][3];
     int has[3];
@@ -63,15 +66,34 @@ struct PfArgs {
 
 // Data type T (float2: complex, float: real), its FP64 counterpart W and
 // the vector P of a lane's two z points
+// and the type F of its Bloch factors (FP32 data takes them rounded to
+// FP32, as the transfer kernels do)
+struct __align__(16) PfD2x2 {
+    double2 a, b;
+};
 template <typename T> struct PfT;
 template <> struct PfT<float2> {
     typedef float4 P;
     typedef double2 W;
+    typedef float2 F;
     static constexpr bool cplx = true;
 };
 template <> struct PfT<float> {
     typedef float2 P;
     typedef double W;
+    typedef float2 F;
+    static constexpr bool cplx = false;
+};
+template <> struct PfT<double2> {
+    typedef PfD2x2 P;
+    typedef double2 W;
+    typedef double2 F;
+    static constexpr bool cplx = true;
+};
+template <> struct PfT<double> {
+    typedef double2 P;
+    typedef double W;
+    typedef double2 F;
     static constexpr bool cplx = false;
 };
 __device__ inline float4 pf_pack(float2 a, float2 b)
@@ -89,10 +111,56 @@ __device__ inline void pf_unpack(float2 u, float& a, float& b)
     a = u.x;
     b = u.y;
 }
+__device__ inline PfD2x2 pf_pack(double2 a, double2 b) { return {a, b}; }
+__device__ inline double2 pf_pack(double a, double b)
+{
+    return make_double2(a, b);
+}
+__device__ inline void pf_unpack(PfD2x2 u, double2& a, double2& b)
+{
+    a = u.a;
+    b = u.b;
+}
+__device__ inline void pf_unpack(double2 u, double& a, double& b)
+{
+    a = u.x;
+    b = u.y;
+}
+// the FP64 residual as the compute type, the result as FP64
+__device__ inline float pf_in(double v) { return (float)v; }
+__device__ inline float2 pf_in(double2 v) { return to_narrow(v); }
+__device__ inline double pf_out(float v) { return (double)v; }
+__device__ inline double2 pf_out(float2 v) { return to_wide(v); }
+__device__ inline double pf_out(double v) { return v; }
+__device__ inline double2 pf_out(double2 v) { return v; }
 
 // Bloch factor times a value (real data: the factor is 1)
 __device__ inline float2 pf_cmul(float2 f, float2 v) { return cmul(f, v); }
 __device__ inline float pf_cmul(float2, float v) { return v; }
+__device__ inline double2 pf_cmul(double2 f, double2 v) { return cmul(f, v); }
+__device__ inline double pf_cmul(double2, double v) { return v; }
+
+// Bloch factors in the precision of the data
+template <typename F> __device__ inline F pf_fac(const PfArgs& a, int c, int d);
+template <>
+__device__ inline float2 pf_fac<float2>(const PfArgs& a, int c, int d)
+{
+    return a.ff[c][d];
+}
+template <>
+__device__ inline double2 pf_fac<double2>(const PfArgs& a, int c, int d)
+{
+    return a.ffd[c][d];
+}
+template <typename F> __device__ inline F pf_cf(const PfArgs& a, int t);
+template <> __device__ inline float2 pf_cf<float2>(const PfArgs& a, int t)
+{
+    return a.cf[t];
+}
+template <> __device__ inline double2 pf_cf<double2>(const PfArgs& a, int t)
+{
+    return a.cfd[t];
+}
 
 // index inside [0, n) after one wrap; false outside a zero boundary
 __device__ inline bool pf_wrap(int& i, int n, int periodic)
@@ -108,6 +176,8 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
 {
     typedef typename PfT<T>::P P;
     typedef typename PfT<T>::W W;
+    typedef typename PfT<T>::F F;
+    typedef typename ScalarOf<T>::type S;
     constexpr bool CPLX = PfT<T>::cplx;
     extern __shared__ __align__(16) unsigned char pf_smem[];
     T* lds = reinterpret_cast<T*>(pf_smem);
@@ -133,8 +203,8 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
     const bool zodd = (a.skip0[2] & 1) != 0;
     // the coarse Bloch factors (complex data; 2 BUF is even, so the table
     // is 8-byte aligned)
-    float2* ctab = reinterpret_cast<float2*>(lds + 2 * BUF);
-    if (CPLX && tid < 27) ctab[tid] = a.cf[tid];
+    F* ctab = reinterpret_cast<F*>(lds + 2 * BUF);
+    if (CPLX && tid < 27) ctab[tid] = pf_cf<F>(a, tid);
     __syncthreads();
 
     // coarse elements of a point (fine row yf inside the grid, z pair t):
@@ -191,7 +261,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         }
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const T h = 0.5f * (v[2 * r] + v[2 * r + 1]);
+            const T h = (S)0.5 * (v[2 * r] + v[2 * r + 1]);
             z[r][0] = zodd ? h : v[2 * r];
             z[r][1] = zodd ? v[2 * r + 1] : h;
         }
@@ -203,7 +273,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         zrows(v, codes, xside, z);
 #pragma unroll
         for (int k = 0; k < 2; k++)
-            Y[k] = two_y ? 0.5f * (z[0][k] + z[1][k]) : z[0][k];
+            Y[k] = two_y ? (S)0.5 * (z[0][k] + z[1][k]) : z[0][k];
     };
 
     // ---- own points: rows y0 + sg PY + j, pair tx.  Two rows per thread
@@ -224,7 +294,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             zrows(v[0], ocode[0], xside, z);
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                const T mid = 0.5f * (z[0][k] + z[1][k]);
+                const T mid = (S)0.5 * (z[0][k] + z[1][k]);
                 Y[0][k] = oty[0] ? mid : z[0][k];
                 Y[1][k] = oty[0] ? z[1][k] : mid;
             }
@@ -311,7 +381,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         for (int j = 0; j < PY; j++)
 #pragma unroll
             for (int k = 0; k < 2; k++)
-                dst[j][k] = -(odd ? 0.5f * (oA[j][k] + oB[j][k]) : oA[j][k]);
+                dst[j][k] = -(odd ? (S)0.5 * (oA[j][k] + oB[j][k]) : oA[j][k]);
     };
     // any fine plane: wrapped with its Bloch factor, zero outside a zero
     // boundary
@@ -326,7 +396,7 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         }
         own_plane(pw, dst);
         if (side) {
-            const float2 f = a.ff[0][side == 1 ? 1 : 0];
+            const F f = pf_fac<F>(a, 0, side == 1 ? 1 : 0);
 #pragma unroll
             for (int j = 0; j < PY; j++)
 #pragma unroll
@@ -379,26 +449,26 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
         for (int m = 0; m < HM; m++) {
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                T v = -(odd ? 0.5f * (hA[m][k] + hB[m][k]) : hA[m][k]);
+                T v = -(odd ? (S)0.5 * (hA[m][k] + hB[m][k]) : hA[m][k]);
                 if (hside[m] == 3) v = zero_of<T>();
                 if (hside[m] == 1 || hside[m] == 2)
-                    v = pf_cmul(a.ff[1][hside[m] == 1 ? 1 : 0], v);
+                    v = pf_cmul(pf_fac<F>(a, 1, hside[m] == 1 ? 1 : 0), v);
                 dst[m][k] = v;
             }
         }
     };
 
-    float cpx[R], cmx[R], cpy[R], cmy[R], cpz[R], cmz[R];
+    S cpx[R], cmx[R], cpy[R], cmy[R], cpz[R], cmz[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        cpx[k] = (float)a.cp[0][k];
-        cmx[k] = (float)a.cm[0][k];
-        cpy[k] = (float)a.cp[1][k];
-        cmy[k] = (float)a.cm[1][k];
-        cpz[k] = (float)a.cp[2][k];
-        cmz[k] = (float)a.cm[2][k];
+        cpx[k] = (S)a.cp[0][k];
+        cmx[k] = (S)a.cm[0][k];
+        cpy[k] = (S)a.cp[1][k];
+        cmy[k] = (S)a.cm[1][k];
+        cpz[k] = (S)a.cp[2][k];
+        cmz[k] = (S)a.cm[2][k];
     }
-    const float c0 = (float)a.c0, ms = -(float)a.step;
+    const S c0 = (S)a.c0, ms = -(S)a.step;
 
     // z padding of the own rows: Bloch mirrors (periodic, every plane) or
     // zeros (written once, both buffers; ordered by the first barrier)
@@ -446,14 +516,14 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             *reinterpret_cast<P*>(row + col) =
                 pf_pack(q[R][j][0], q[R][j][1]);
             if (zmir && zhi) {
-                const T u0 = pf_cmul(a.ff[2][0], q[R][j][0]);
-                const T u1 = pf_cmul(a.ff[2][0], q[R][j][1]);
+                const T u0 = pf_cmul(pf_fac<F>(a, 2, 0), q[R][j][0]);
+                const T u1 = pf_cmul(pf_fac<F>(a, 2, 0), q[R][j][1]);
                 *reinterpret_cast<P*>(row + col + n2) =
                     pf_pack(u0, u1);
             }
             if (zmir && zlo) {
-                const T u0 = pf_cmul(a.ff[2][1], q[R][j][0]);
-                const T u1 = pf_cmul(a.ff[2][1], q[R][j][1]);
+                const T u0 = pf_cmul(pf_fac<F>(a, 2, 1), q[R][j][0]);
+                const T u1 = pf_cmul(pf_fac<F>(a, 2, 1), q[R][j][1]);
                 *reinterpret_cast<P*>(row + col - n2) =
                     pf_pack(u0, u1);
             }
@@ -533,9 +603,12 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
             for (int k = 0; k < 2; k++) {
                 const T v = q[R][j][k];
                 T r = fmac(acc[k], c0, v);
-                r = r - to_narrow(rc[j][k]);
+                if constexpr (std::is_same<T, W>::value)
+                    r = r - rc[j][k];
+                else
+                    r = r - pf_in(rc[j][k]);
                 const T d = fmac(v, ms, r);
-                o[k] = to_wide(-d);
+                o[k] = pf_out(-d);
             }
             if (a.nt_out) {
                 // streamed result: non-temporal stores
@@ -623,7 +696,7 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                         const int fine_periodic[3], const XferPlan& ip,
                         const int coarse_periodic[3], const double* phases,
                         const void* d1, const void* res, void* out, long nin,
-                        double step, int real, hipStream_t s)
+                        double step, int real, int fp64, hipStream_t s)
 {
     if (!g_pf_on || !ip.interpolate || ip.K != 2 || ax.R < 1 || ax.R > 3)
         return ENOTSUP_;
@@ -634,13 +707,22 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
     if (n2 % 2 || n2 / 2 < 4 || n2 / 2 > PF_MAXT || n0 < R || n1 < R)
         return ENOTSUP_;
     const int NZ = n2 / 2;
-    // tile: TY = NYS PY dividing n1, the tallest of at most 256 threads
+    // LDS of a tile TY rows high: two plane buffers of T, then the 27
+    // complex factors (aligned: the buffers hold an even number of
+    // elements)
+    const size_t es = (real ? 4 : 8) * (fp64 ? 2 : 1);
+    auto lds_of = [&](int ty) {
+        return 2 * (size_t)(ty + 2 * R) * (n2 + 2 * PF_RP) * es + 28 * 16;
+    };
+    // tile: TY = NYS PY dividing n1, the tallest of at most 512 threads
+    // whose LDS fits 64 KB
     int PY = 0, NYS = 0;
     for (int py : {2, 1}) {
         if (g_pf_py && py != g_pf_py) continue;
         for (int nys = PF_MAXT / NZ; nys >= 1; nys--) {
             if (g_pf_nys && nys != g_pf_nys) continue;
-            if (n1 % (nys * py) == 0 && nys * py >= R) {
+            if (n1 % (nys * py) == 0 && nys * py >= R &&
+                lds_of(nys * py) <= 64 * 1024) {
                 PY = py;
                 NYS = nys;
                 break;
@@ -649,6 +731,10 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
         if (PY) break;
     }
     const bool forced = g_pf_py || g_pf_nys;
+    // complex FP64 data: 235-256 VGPRs leave the fused pass no faster than
+    // the two passes (6.27 ms for both on 128 bands of 128^3), so it runs
+    // only on a forced tile
+    if (fp64 && !real && !forced) return ENOTSUP_;
     const int TY = NYS * PY, NT = NZ * NYS;
     const int H = 2 * R * NZ;
     const int HM = PY ? (H + NT - 1) / NT : 0;
@@ -659,11 +745,7 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                   g_pf_py, g_pf_nys, n0, n1, n2);
         return EINVAL_;
     }
-    // two plane buffers of T, then the 27 complex factors (8-byte aligned)
-    const size_t es = real ? 4 : 8;
-    const size_t lds =
-        2 * (size_t)(TY + 2 * R) * (n2 + 2 * PF_RP) * es + 28 * 8;
-    if (lds > 64 * 1024) return ENOTSUP_;
+    const size_t lds = lds_of(TY);
     const long cplane = (long)ip.nin[1] * ip.nin[2];
     if (cplane * (long)es >= 0x7fffffffL) return ENOTSUP_;
 
@@ -685,6 +767,7 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                                              phases[4 * c + 2 * d + 1])
                               : make_double2(1.0, 0.0);
             a.ff[c][d] = make_float2((float)ph[c][d].x, (float)ph[c][d].y);
+            a.ffd[c][d] = ph[c][d];
             if ((fine_periodic[c] || coarse_periodic[c]) &&
                 (ph[c][d].x != 1.0 || ph[c][d].y != 0.0))
                 a.phased = 1;
@@ -702,6 +785,7 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
                 if (ys) f = cmul(f, ph[1][ys == 1 ? 1 : 0]);
                 if (zs) f = cmul(f, ph[2][zs == 1 ? 1 : 0]);
                 a.cf[9 * xs + ys + 3 * zs] = make_float2((float)f.x, (float)f.y);
+                a.cfd[9 * xs + ys + 3 * zs] = f;
             }
     a.c0 = c0;
     a.step = step;
@@ -720,6 +804,9 @@ int launch_precond_fine(const AxisStencil& ax, double c0, const int n[3],
     if (nb > 0x7fffffffL) return ENOTSUP_;
     a.nblocks = (int)nb;
 
+    if (fp64)
+        return real ? pf_dispatch<double>(a, R, PY, HM, NZ, NYS, lds, s)
+                    : pf_dispatch<double2>(a, R, PY, HM, NZ, NYS, lds, s);
     return real ? pf_dispatch<float>(a, R, PY, HM, NZ, NYS, lds, s)
                 : pf_dispatch<float2>(a, R, PY, HM, NZ, NYS, lds, s);
 }
