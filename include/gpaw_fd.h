@@ -466,9 +466,9 @@ int gfd_woperator_get_diagonal_element(gfd_woperator* op, double* out,
  * tiles, or a tile code nt | py << 12 | zmax << 16 | xc << 28 with nt >= 64
  * forcing that x-march tile) sets the batched transfers of batches tuned
  * afterwards; "precond_fine" (0/1/2, default 1) runs the fine level of
- * the preconditioner chains (gfd_fd_precond32 real or complex,
- * gfd_fd_precond real; complex FP64 only with a forced tile) as one fused
- * pass where the shape fits (0: interpolation and update as two passes;
+ * the preconditioner chains (gfd_fd_precond32 and gfd_fd_precond, real
+ * or complex) as one fused pass where the shape fits (0: interpolation and
+ * update as two passes;
  * 2: fused, with non-temporal stores of the result);
  * "precond_fine_tile" (py | nys << 8 | xc << 16, 0 automatic) forces its
  * tile: py rows per thread (1, 2), nys thread rows, xc x planes per block
