@@ -403,7 +403,8 @@ def subspace_step(gd, kin, vt, phase, psi, hpsi, out):
 
 def stage_precond_128(dev, nbands=128, reps=5):
     """The FP32 kinetic preconditioner on one 128-band block of the
-    1024-band workload (128^3 c128, first k-point): reps applications.
+    1024-band workload (128^3 c128, first k-point): reps applications,
+    then the same with the FP64 chain.
     Traffic model per fine point: complex128 residual read twice and result
     written once (48 B; the fine level is one fused pass, its interpolated
     d0' never stored), the coarse levels ~1/7 of the two-pass fine work
@@ -427,10 +428,16 @@ def stage_precond_128(dev, nbands=128, reps=5):
     ms, _ = timed(lambda: P(R, K, out=out), reps, 1)
     pts = nbands * 128**3
     gb = pts * (48 + 64 / 7) / 1e9
+    # the FP64 chain (the reference's precision) on the same block
+    del P
+    P64 = Preconditioner(gd, kin, complex, block=nbands)
+    ms64, _ = timed(lambda: P64(R, K, out=out), reps, 1)
     return {'ms': round(ms, 3), 'bands': nbands,
             'ms_per_1024_bands': round(ms * 1024 / nbands, 2),
             'GPts/s': round(pts / ms / 1e6, 1), 'model_GB': round(gb, 2),
-            'hbm_frac_model': round(gb / ms / 8.0, 4)}
+            'hbm_frac_model': round(gb / ms / 8.0, 4),
+            'fp64_chain_ms': round(ms64, 3),
+            'fp64_chain_ms_per_1024_bands': round(ms64 * 1024 / nbands, 2)}
 
 
 def stage_density_96(dev, reps=20):
@@ -771,7 +778,8 @@ def main():
         ms_p, _ = timed(lambda: P(res, Kpt, out=out), 5, 1)
         stages['precond_c128_32x128'] = {
             'ms': round(ms_p, 3),
-            'GPts/s': round(32 * 128**3 / ms_p / 1e6, 1)}
+            'GPts/s': round(32 * 128**3 / ms_p / 1e6, 1),
+            'chain': 'FP64 (the reference precision), fused fine level'}
         # weighted operator of the dielectric Poisson solver: eps Laplace +
         # grad(eps) . Gradient (nn=3) on 16 densities of 96^3, weights shared
         from gpaw_amd.fd_operators import Gradient
