@@ -204,7 +204,8 @@ __global__ void __launch_bounds__(MAXT) k_pfine(PfArgs a)
     // the coarse Bloch factors (complex data; 2 BUF is even, so the table
     // is 8-byte aligned)
     F* ctab = reinterpret_cast<F*>(lds + 2 * BUF);
-    if (CPLX && tid < 27) ctab[tid] = pf_cf<F>(a, tid);
+    if (CPLX)
+        for (int t = tid; t < 27; t += NT) ctab[t] = pf_cf<F>(a, t);
     __syncthreads();
 
     // coarse elements of a point (fine row yf inside the grid, z pair t):

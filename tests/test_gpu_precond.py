@@ -81,7 +81,8 @@ def test_fp32_chain(dtype, pbc, path):
 FUSED_CASES = [((32, 24, 48), True, 0), ((16, 20, 24), False, 0),
                ((24, 16, 32), (True, False, True), 0),
                ((20, 28, 16), True, 1 | 4 << 8),
-               ((28, 16, 24), (False, True, True), 2 | 4 << 8 | 5 << 16)]
+               ((28, 16, 24), (False, True, True), 2 | 4 << 8 | 5 << 16),
+               ((8, 12, 8), True, 0), ((12, 8, 40), (True, True, False), 0)]
 
 
 @pytest.mark.parametrize('fp32', [True, False])
